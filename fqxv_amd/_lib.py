@@ -1,0 +1,88 @@
+"""ctypes binding of libfqxv_rans_hip.so (include/fqxv_rans_hip.h).
+
+The library is built in-tree by ``python -m fqxv_amd.build_lib`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
+is no CPU fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfqxv_rans_hip.so")
+
+OK = 0
+ERR_TRUNCATED = 1
+ERR_UNKNOWN_ORDER = 2
+ERR_FREQ_EXCEEDS = 3
+ERR_FREQ_SUM = 4
+ERR_OUTPUT_ALLOC = 5
+ERR_ABSENT_CONTEXT = 6
+ERR_BOUND = 7
+ERR_DST_TOO_SMALL = 8
+ERR_INVALID_ARG = 9
+ERR_DEVICE = 10
+
+# Every symbol include/fqxv_rans_hip.h declares.
+EXPORTS = [
+    "fqxv_rans_strerror", "fqxv_rans_encode_bound", "fqxv_rans_peek_len",
+    "fqxv_rans_ctx_create", "fqxv_rans_ctx_destroy", "fqxv_rans_encode",
+    "fqxv_rans_decode", "fqxv_rans_decode_bounded", "fqxv_rans_decode_batch",
+    "fqxv_rans_encode_batch", "fqxv_rans_decode_one_device",
+    "fqxv_rans_encode_one_device", "fqxv_rans_lpt_order", "fqxv_rans_max_waves",
+]
+
+
+class Job(ctypes.Structure):
+    """fqxv_rans_job (48 bytes)."""
+    _fields_ = [("src_off", ctypes.c_uint64), ("src_len", ctypes.c_uint64),
+                ("dst_off", ctypes.c_uint64), ("dst_cap", ctypes.c_uint64),
+                ("max_len", ctypes.c_uint64), ("order", ctypes.c_int32),
+                ("flags", ctypes.c_uint32)]
+
+
+JOB_DTYPE_FIELDS = [("src_off", "<u8"), ("src_len", "<u8"), ("dst_off", "<u8"),
+                    ("dst_cap", "<u8"), ("max_len", "<u8"), ("order", "<i4"),
+                    ("flags", "<u4")]
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the library (raises OSError when it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built; run python -m fqxv_amd.build_lib")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        sz = ctypes.c_size_t
+        L.fqxv_rans_strerror.restype = ctypes.c_char_p
+        L.fqxv_rans_strerror.argtypes = [ctypes.c_int]
+        L.fqxv_rans_encode_bound.restype = sz
+        L.fqxv_rans_encode_bound.argtypes = [sz, ctypes.c_int]
+        L.fqxv_rans_peek_len.argtypes = [vp, sz, ctypes.POINTER(ctypes.c_uint64)]
+        L.fqxv_rans_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.fqxv_rans_ctx_destroy.argtypes = [vp]
+        L.fqxv_rans_ctx_destroy.restype = None
+        L.fqxv_rans_encode.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz, ctypes.POINTER(sz)]
+        L.fqxv_rans_decode.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(sz)]
+        L.fqxv_rans_decode_bounded.argtypes = [vp, vp, sz, sz, vp, sz, ctypes.POINTER(sz)]
+        for fn in (L.fqxv_rans_decode_batch, L.fqxv_rans_encode_batch):
+            fn.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp, vp]
+        for fn in (L.fqxv_rans_decode_one_device, L.fqxv_rans_encode_one_device):
+            fn.argtypes = [vp, vp, vp, ctypes.POINTER(Job), vp, vp, vp]
+        L.fqxv_rans_lpt_order.argtypes = [vp, ctypes.c_uint32, vp]
+        L.fqxv_rans_lpt_order.restype = None
+        L.fqxv_rans_max_waves.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+        _lib = L
+        return L
+
+
+def strerror(code: int) -> str:
+    return load().fqxv_rans_strerror(code).decode()

@@ -1,0 +1,351 @@
+// C ABI of the gfx950 rANS Nx16 coder (declared in include/fqxv_rans_hip.h).
+//
+// Mirrors crates/fqxv-rans/src/lib.rs: encode (lib.rs:111-127), decode
+// (lib.rs:186-200), peek_output_len / decode_bounded (lib.rs:207-231).  The
+// host-buffer entry points stage through device memory owned by the context;
+// the batched entry points work on caller-owned device arenas.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "../../include/fqxv_rans_hip.h"
+
+typedef uint8_t u8;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int32_t i32;
+
+// Device-side launchers (rans_decode.hip / rans_encode.hip).
+hipError_t fqxv_launch_decode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32, const u32 *, u32 *, i32 *,
+                                    u64 *, u8 *, u32, hipStream_t);
+hipError_t fqxv_launch_decode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
+hipError_t fqxv_decode_occupancy(int *);
+hipError_t fqxv_launch_encode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32, const u32 *, u32 *, i32 *,
+                                    u64 *, u8 *, u32, hipStream_t);
+hipError_t fqxv_launch_encode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
+hipError_t fqxv_encode_occupancy(int *);
+
+namespace {
+
+constexpr u64 WS_PER_WAVE = 256ull * 4096 * 2 + 256ull * 256 * 4; // matches rans_device.h
+constexpr u32 N_COUNTERS = 64;
+
+u64 hdr_bound(u64 n, int order)
+{
+    if (n == 0) return 9;
+    const u64 k = n < 256 ? n : 256;
+    return order == 0 ? 9 + 512 + 128 : 9 + 32 + 512 * k + 128;
+}
+
+} // namespace
+
+struct fqxv_rans_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    u32 dec_waves = 0, enc_waves = 0;
+    u8 *ws = nullptr;
+    u64 ws_waves = 0;
+    u32 *counters = nullptr;
+    u32 next_counter = 0;
+    u8 *d_in = nullptr;
+    u64 d_in_cap = 0;
+    u8 *d_out = nullptr;
+    u64 d_out_cap = 0;
+    i32 *d_status = nullptr;
+    u64 *d_out_len = nullptr;
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+int ensure_ws(fqxv_rans_ctx *c, u64 waves)
+{
+    if (c->ws_waves >= waves) return FQXV_RANS_OK;
+    if (c->ws) {
+        if (hipDeviceSynchronize() != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+        (void)hipFree(c->ws);
+        c->ws = nullptr;
+        c->ws_waves = 0;
+    }
+    if (hipMalloc((void **)&c->ws, waves * WS_PER_WAVE) != hipSuccess) {
+        c->ws = nullptr;
+        return FQXV_RANS_ERR_DEVICE;
+    }
+    c->ws_waves = waves;
+    return FQXV_RANS_OK;
+}
+
+int ensure_buf(u8 **p, u64 *cap, u64 need)
+{
+    need = std::max<u64>(need, 64);
+    if (*cap >= need) return FQXV_RANS_OK;
+    if (*p) {
+        if (hipDeviceSynchronize() != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+        (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
+    // Round up to limit reallocations; the slack also keeps aligned reads
+    // near the end of a stream inside the allocation.
+    u64 sz = 1;
+    while (sz < need + 64) sz <<= 1;
+    if (hipMalloc((void **)p, sz) != hipSuccess) {
+        *p = nullptr;
+        return FQXV_RANS_ERR_DEVICE;
+    }
+    *cap = sz;
+    return FQXV_RANS_OK;
+}
+
+hipStream_t pick_stream(fqxv_rans_ctx *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
+
+u32 *next_counter(fqxv_rans_ctx *c)
+{
+    u32 *p = c->counters + (c->next_counter % N_COUNTERS);
+    c->next_counter++;
+    return p;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *fqxv_rans_strerror(int code)
+{
+    switch (code) {
+    case FQXV_RANS_OK: return "ok";
+    case FQXV_RANS_ERR_TRUNCATED: return "truncated stream";
+    case FQXV_RANS_ERR_UNKNOWN_ORDER: return "unknown order tag";
+    case FQXV_RANS_ERR_FREQ_EXCEEDS: return "rANS freq table exceeds TOTFREQ";
+    case FQXV_RANS_ERR_FREQ_SUM: return "rANS freq table does not sum to TOTFREQ";
+    case FQXV_RANS_ERR_OUTPUT_ALLOC: return "rANS output length too large to allocate";
+    case FQXV_RANS_ERR_ABSENT_CONTEXT: return "symbol references absent context";
+    case FQXV_RANS_ERR_BOUND: return "rANS output length exceeds caller bound";
+    case FQXV_RANS_ERR_DST_TOO_SMALL: return "destination buffer too small";
+    case FQXV_RANS_ERR_INVALID_ARG: return "invalid argument";
+    case FQXV_RANS_ERR_DEVICE: return "HIP device error";
+    default: return "unknown error";
+    }
+}
+
+size_t fqxv_rans_encode_bound(size_t n, int order)
+{
+    if (n == 0) return 9;
+    return (size_t)(hdr_bound(n, order) + 2 * (u64)n + 16);
+}
+
+int fqxv_rans_peek_len(const uint8_t *src, size_t len, uint64_t *n)
+{
+    if (!src || len < 9) return FQXV_RANS_ERR_TRUNCATED;
+    u64 v = 0;
+    for (int i = 0; i < 8; i++) v |= (u64)src[1 + i] << (8 * i);
+    if (n) *n = v;
+    return FQXV_RANS_OK;
+}
+
+int fqxv_rans_ctx_create(int device, fqxv_rans_ctx **out)
+{
+    if (!out) return FQXV_RANS_ERR_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return FQXV_RANS_ERR_DEVICE;
+    DeviceGuard g(device);
+    auto *c = new (std::nothrow) fqxv_rans_ctx();
+    if (!c) return FQXV_RANS_ERR_DEVICE;
+    c->device = device;
+    hipDeviceProp_t prop;
+    int dec_occ = 0, enc_occ = 0;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || fqxv_decode_occupancy(&dec_occ) != hipSuccess ||
+        fqxv_encode_occupancy(&enc_occ) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipMalloc((void **)&c->counters, N_COUNTERS * sizeof(u32)) != hipSuccess ||
+        hipMalloc((void **)&c->d_status, sizeof(i32)) != hipSuccess ||
+        hipMalloc((void **)&c->d_out_len, sizeof(u64)) != hipSuccess) {
+        fqxv_rans_ctx_destroy(c);
+        return FQXV_RANS_ERR_DEVICE;
+    }
+    c->dec_waves = (u32)std::max(1, dec_occ) * (u32)prop.multiProcessorCount;
+    c->enc_waves = (u32)std::max(1, enc_occ) * (u32)prop.multiProcessorCount;
+    *out = c;
+    return FQXV_RANS_OK;
+}
+
+void fqxv_rans_ctx_destroy(fqxv_rans_ctx *c)
+{
+    if (!c) return;
+    {
+        DeviceGuard g(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        (void)hipDeviceSynchronize();
+        if (c->ws) (void)hipFree(c->ws);
+        if (c->counters) (void)hipFree(c->counters);
+        if (c->d_in) (void)hipFree(c->d_in);
+        if (c->d_out) (void)hipFree(c->d_out);
+        if (c->d_status) (void)hipFree(c->d_status);
+        if (c->d_out_len) (void)hipFree(c->d_out_len);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+int fqxv_rans_max_waves(fqxv_rans_ctx *c, uint32_t *waves)
+{
+    if (!c || !waves) return FQXV_RANS_ERR_INVALID_ARG;
+    *waves = std::max(c->dec_waves, c->enc_waves);
+    return FQXV_RANS_OK;
+}
+
+void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out)
+{
+    std::iota(order_out, order_out + count, 0u);
+    std::stable_sort(order_out, order_out + count, [&](u32 a, u32 b) { return sizes[a] > sizes[b]; });
+}
+
+int fqxv_rans_decode_bounded(fqxv_rans_ctx *c, const uint8_t *src, size_t len, size_t max_len, uint8_t *dst,
+                             size_t cap, size_t *out_len)
+{
+    if (out_len) *out_len = 0;
+    if (!c || (!src && len) || !out_len) return FQXV_RANS_ERR_INVALID_ARG;
+    u64 n = 0;
+    const bool has_n = fqxv_rans_peek_len(src, len, &n) == FQXV_RANS_OK;
+    if (has_n && n > (u64)max_len) return FQXV_RANS_ERR_BOUND;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    const u64 dcap = has_n ? std::min<u64>(n, (u64)cap) : 0;
+    if (has_n && n <= (u64)cap && n > 0 && !dst) return FQXV_RANS_ERR_INVALID_ARG;
+    int rc;
+    if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dcap)) ||
+        (rc = ensure_ws(c, 1)))
+        return rc;
+    if (len && hipMemcpyAsync(c->d_in, src, len, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    fqxv_rans_job jb{0, (u64)len, 0, dcap, UINT64_MAX, 0, 0};
+    if (fqxv_launch_decode_one(c->d_in, c->d_out, jb, c->d_status, c->d_out_len, c->ws, c->stream) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    i32 st = 0;
+    u64 ol = 0;
+    if (hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(&ol, c->d_out_len, sizeof ol, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    if (st != FQXV_RANS_OK) return st;
+    if (ol && hipMemcpy(dst, c->d_out, ol, hipMemcpyDeviceToHost) != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+    *out_len = (size_t)ol;
+    return FQXV_RANS_OK;
+}
+
+int fqxv_rans_decode(fqxv_rans_ctx *c, const uint8_t *src, size_t len, uint8_t *dst, size_t cap, size_t *out_len)
+{
+    return fqxv_rans_decode_bounded(c, src, len, SIZE_MAX, dst, cap, out_len);
+}
+
+int fqxv_rans_encode(fqxv_rans_ctx *c, const uint8_t *src, size_t n, int order, uint8_t *dst, size_t cap,
+                     size_t *out_len)
+{
+    if (out_len) *out_len = 0;
+    if (!c || (!src && n) || !out_len || !dst) return FQXV_RANS_ERR_INVALID_ARG;
+    if (order != 0 && order != 1) return FQXV_RANS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    const u64 bound = fqxv_rans_encode_bound(n, order);
+    int rc;
+    if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, n)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, bound)) ||
+        (rc = ensure_ws(c, 1)))
+        return rc;
+    if (n && hipMemcpyAsync(c->d_in, src, n, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    fqxv_rans_job jb{0, (u64)n, 0, bound, 0, order, 0};
+    if (fqxv_launch_encode_one(c->d_in, c->d_out, jb, c->d_status, c->d_out_len, c->ws, c->stream) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    i32 st = 0;
+    u64 ol = 0;
+    if (hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(&ol, c->d_out_len, sizeof ol, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    if (st != FQXV_RANS_OK) return st;
+    if (ol > cap) return FQXV_RANS_ERR_DST_TOO_SMALL;
+    if (hipMemcpy(dst, c->d_out, ol, hipMemcpyDeviceToHost) != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+    *out_len = (size_t)ol;
+    return FQXV_RANS_OK;
+}
+
+static int batch_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
+                        uint32_t count, const uint32_t *d_order, int32_t *d_status, uint64_t *d_out_len,
+                        void *stream, bool enc)
+{
+    if (!c) return FQXV_RANS_ERR_INVALID_ARG;
+    if (count == 0) return FQXV_RANS_OK;
+    if (!d_src || !d_dst || !d_jobs || !d_status || !d_out_len) return FQXV_RANS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    u32 grid = std::min<u32>(count, enc ? c->enc_waves : c->dec_waves);
+    int rc = ensure_ws(c, grid);
+    if (rc) return rc;
+    hipStream_t st = pick_stream(c, stream);
+    u32 *ctr = next_counter(c);
+    const hipError_t e = enc ? fqxv_launch_encode_batch(d_src, d_dst, d_jobs, count, d_order, ctr, d_status,
+                                                        d_out_len, c->ws, grid, st)
+                             : fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, ctr, d_status,
+                                                        d_out_len, c->ws, grid, st);
+    return e == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
+}
+
+int fqxv_rans_decode_batch(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
+                           uint32_t count, const uint32_t *d_order, int32_t *d_status, uint64_t *d_out_len,
+                           void *stream)
+{
+    return batch_common(c, d_src, d_dst, d_jobs, count, d_order, d_status, d_out_len, stream, false);
+}
+
+int fqxv_rans_encode_batch(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
+                           uint32_t count, const uint32_t *d_order, int32_t *d_status, uint64_t *d_out_len,
+                           void *stream)
+{
+    return batch_common(c, d_src, d_dst, d_jobs, count, d_order, d_status, d_out_len, stream, true);
+}
+
+static int one_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,
+                      int32_t *d_status, uint64_t *d_out_len, void *stream, bool enc)
+{
+    if (!c || !job || !d_src || !d_dst || !d_status || !d_out_len) return FQXV_RANS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    int rc = ensure_ws(c, 1);
+    if (rc) return rc;
+    hipStream_t st = pick_stream(c, stream);
+    const hipError_t e = enc ? fqxv_launch_encode_one(d_src, d_dst, *job, d_status, d_out_len, c->ws, st)
+                             : fqxv_launch_decode_one(d_src, d_dst, *job, d_status, d_out_len, c->ws, st);
+    return e == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
+}
+
+int fqxv_rans_decode_one_device(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,
+                                int32_t *d_status, uint64_t *d_out_len, void *stream)
+{
+    return one_common(c, d_src, d_dst, job, d_status, d_out_len, stream, false);
+}
+
+int fqxv_rans_encode_one_device(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,
+                                int32_t *d_status, uint64_t *d_out_len, void *stream)
+{
+    return one_common(c, d_src, d_dst, job, d_status, d_out_len, stream, true);
+}
+
+} // extern "C"

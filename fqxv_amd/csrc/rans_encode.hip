@@ -1,0 +1,486 @@
+// gfx950 rANS Nx16 encoder (order-0 and order-1).
+//
+// Byte-identical to crates/fqxv-rans/src/scalar.rs:29-140 (encode_order0,
+// encode_order1, encode_symbol, finish) with normalize / EncSym from
+// model.rs:70-264.
+//
+// One workgroup = one wave64 encodes one block at a time (persistent grid,
+// device work queue).  Per block:
+//   1. histogram with LDS atomics (order-1: dense-remapped pair counts in
+//      LDS when the alphabet has <= 64 symbols, global atomics otherwise);
+//   2. normalize (model.rs:213-264) as wave reductions -- the first index with
+//      the strictly greatest frequency absorbs the residual, exactly as the
+//      reference's sequential scan does;
+//   3. per (context, symbol) constants: f << 12 | cum and the reciprocal
+//      magic = floor((2^32-1)/f), which gives q0 in {q-1, q} for every
+//      32-bit state, so one correction yields the exact v / f (any exact
+//      division gives the reference's bytes, model.rs:313-365);
+//   4. the reverse walk: lanes 0..31 hold the states; rounds run from the
+//      last to the first; a ballot + v_mbcnt rank places each lane's
+//      renormalisation word (at most one per step, scalar.rs:120-127).  The
+//      reference collects words in descending position order and reverses
+//      them (scalar.rs:130-140); here each word is written straight to its
+//      final offset counted back from the end of the output region, then the
+//      word block is slid down behind the states.
+// The input of the walk is staged through LDS 2 KiB at a time, double
+// buffered, so the state chain never waits on HBM.
+#include "rans_device.h"
+
+namespace {
+
+constexpr u32 DENSE_COLS = 64;
+constexpr u32 DENSE_ROWS = 65;           // 64 dense symbols + context 0 when absent
+constexpr u32 TAB_ENTRIES = DENSE_ROWS * DENSE_COLS;
+constexpr u32 CH = 2048;                 // walk staging chunk (positions)
+constexpr u32 CH_DW = CH / 4 + 1;        // + 4 leading bytes (context of the first)
+
+struct EncShared {
+    u32 fc[TAB_ENTRIES];    // counts, then f << 12 | cum
+    u32 mg[TAB_ENTRIES];    // reciprocal magic per entry
+    u32 hist[256];
+    u8 colmap[256];         // symbol -> dense column (order-1 dense mode)
+    u8 rowmap[256];         // context -> dense row
+    u8 ctx_of[256];         // k-th present context (ascending)
+    u32 stage[2][CH_DW + 3];
+};
+
+// Bytes of p[0..3] that fall in [begin, end), others zero.
+__device__ __forceinline__ u32 load4b(const u8 *p, const u8 *begin, const u8 *end)
+{
+    if (p >= begin) return load4(p, end);
+    u32 v = 0;
+    for (u32 i = 0; i < 4; i++)
+        if (p + i >= begin && p + i < end) v |= (u32)p[i] << (8 * i);
+    return v;
+}
+
+// floor(c * 4096 / total), exact, via double with one correction each way.
+__device__ __forceinline__ u32 scaled(u32 c, u64 total)
+{
+    const u64 num = (u64)c << SCALE_BITS;
+    u64 q = (u64)((double)num / (double)total);
+    if (q * total > num) q--;
+    if ((q + 1) * total <= num) q++;
+    return (u32)q;
+}
+
+__device__ __forceinline__ u32 pick4(const u32 v[4], u32 i)
+{
+    return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+
+// model.rs:213-264 over 256 counts, lane l owning symbols 4l..4l+3.
+__device__ void normalize_wave(const u32 c[4], u32 f[4])
+{
+    const u32 l = lane_id();
+    const u64 total = wave_sum64((u64)c[0] + c[1] + c[2] + c[3]);
+    for (int i = 0; i < 4; i++) f[i] = 0;
+    if (total == 0) return;
+    u32 bv = 0, bk = 0xffffffffu;
+    for (int i = 0; i < 4; i++) {
+        if (c[i] == 0) continue;
+        u32 v = scaled(c[i], total);
+        if (v == 0) v = 1;
+        f[i] = v;
+        if (v > bv) {
+            bv = v;
+            bk = 4 * l + i;
+        }
+    }
+    const u32 sum = wave_sum(f[0] + f[1] + f[2] + f[3]);
+    wave_argmax(bv, bk);
+    u32 max_s = uni(bk);
+    int diff = (int)TOTFREQ - (int)sum;
+    while (diff != 0) {
+        const u32 owner = max_s >> 2, idx = max_s & 3;
+        const u32 cur = uni(__shfl(pick4(f, idx), owner));
+        if (diff > 0) {
+            if (l == owner) {
+                for (int i = 0; i < 4; i++)
+                    if ((u32)i == idx) f[i] = cur + (u32)diff;
+            }
+            diff = 0;
+        } else {
+            int take = (int)cur - 1;
+            if (take < 0) take = 0;
+            if (take > -diff) take = -diff;
+            if (l == owner) {
+                for (int i = 0; i < 4; i++)
+                    if ((u32)i == idx) f[i] = cur - (u32)take;
+            }
+            diff += take;
+            if (diff != 0) {
+                // First index with the strictly greatest frequency above 1.
+                u32 v2 = 1, k2 = 0xffffffffu;
+                for (int i = 0; i < 4; i++)
+                    if (f[i] > v2) {
+                        v2 = f[i];
+                        k2 = 4 * l + i;
+                    }
+                wave_argmax(v2, k2);
+                max_s = uni(k2);
+                if (max_s == 0xffffffffu) return; // unreachable for sum > TOTFREQ
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ u32 magic_of(u32 f) { return f ? 0xffffffffu / f : 0u; }
+
+// Write 256 freqs (lane-distributed) as little-endian u16 at p.
+__device__ __forceinline__ void put_freqs(u8 *p, const u32 f[4])
+{
+    const u32 l = lane_id();
+    u8 *q = p + 8 * l;
+    for (int i = 0; i < 4; i++) {
+        q[2 * i] = (u8)f[i];
+        q[2 * i + 1] = (u8)(f[i] >> 8);
+    }
+}
+
+// Stage positions [p0 - 4, p0 + CH) of `in` (bytes outside [in, in+n) = 0).
+__device__ __forceinline__ void stage_fetch(const u8 *in, u64 n, u64 p0, u32 (&r)[9])
+{
+    const u8 *g = in + p0 - 4;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const u32 t = lane_id() + 64 * i;
+        r[i] = t < CH_DW ? load4b(g + 4 * t, in, in + n) : 0u;
+    }
+}
+
+__device__ __forceinline__ void stage_put(u32 *buf, const u32 (&r)[9])
+{
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const u32 t = lane_id() + 64 * i;
+        if (t < CH_DW) buf[t] = r[i];
+    }
+}
+
+// Encode-walk table access.
+struct TabRef {
+    const u32 *fc;
+    const u32 *mg;
+    const u8 *rowmap;
+    const u8 *colmap;
+    u32 mode;   // 0: order-0 (index = sym); 1: dense order-1; 2: global order-1
+};
+
+__device__ __forceinline__ u32 tab_index(const TabRef &t, u32 ctx, u32 sym)
+{
+    if (t.mode == 0) return sym;
+    if (t.mode == 1) return (u32)t.rowmap[ctx] * DENSE_COLS + t.colmap[sym];
+    return ctx * 256 + sym;
+}
+
+// The reverse walk over all n positions; returns the number of words.
+__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u32 &x)
+{
+    const u32 l = lane_id();
+    x = RANS_L;
+    u64 cnt = 0;
+    const u64 nch = (n + CH - 1) / CH;
+    u32 pf[9];
+    stage_fetch(in, n, (nch - 1) * CH, pf);
+    stage_put(sh.stage[(nch - 1) & 1], pf);
+    __syncthreads();
+    for (u64 ch = nch; ch-- > 0;) {
+        const u64 p0 = ch * CH;
+        if (ch > 0) stage_fetch(in, n, p0 - CH, pf);
+        const u8 *sb = (const u8 *)sh.stage[ch & 1];
+        const u64 pend = p0 + CH < n ? p0 + CH : n;
+        const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
+        for (u64 r = r_hi + 1; r-- > r_lo;) {
+            const u64 i = r * NSTATES + l;
+            const bool act = l < NSTATES && i < n;
+            const u32 t = (u32)(i - p0);
+            const u32 sym = act ? sb[4 + t] : 0u;
+            const u32 ctx = (act && i > 0) ? sb[3 + t] : 0u;
+            const u32 e = tab_index(tr, ctx, sym);
+            const u32 fcm = act ? tr.fc[e] : (TOTFREQ << 12);
+            const u32 mg = act ? tr.mg[e] : 1u;
+            const u32 f = fcm >> 12, cum = fcm & 0xfff;
+            // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73).
+            const bool need = act && (x >> 20) >= f;
+            const u64 m = ballot(need);
+            const u32 k = (u32)__popcll(m);
+            if (need) {
+                // Emission index in the reference's renorm vector: higher
+                // lanes of the round come first (descending positions).
+                const u64 idx = cnt + (k - 1 - rank_below(m));
+                u8 *w = wend - 2 * (idx + 1);
+                w[0] = (u8)x;
+                w[1] = (u8)(x >> 8);
+            }
+            const u32 v = need ? (x >> 16) : x;
+            u32 q = __umulhi(v, mg);
+            u32 rr = v - q * f;
+            if (rr >= f) {
+                q++;
+                rr -= f;
+            }
+            if (act) x = (q << SCALE_BITS) + rr + cum;
+            cnt += k;
+        }
+        if (ch > 0) {
+            __syncthreads();
+            stage_put(sh.stage[(ch - 1) & 1], pf);
+            __syncthreads();
+        }
+    }
+    return cnt;
+}
+
+// Tail of every encode: states, then the words slid down behind them.
+__device__ void finish(u8 *dst, u64 hdr, u8 *wend, u64 W, u32 x)
+{
+    const u32 l = lane_id();
+    if (l < NSTATES) store_bytes(dst + hdr - 4 * NSTATES + 4 * l, x, 4);
+    // Words were stored by other lanes; order them before the copy reads.
+    __threadfence();
+    wave_copy_fwd(dst + hdr, wend - 2 * W, 2 * W);
+}
+
+// Histogram of in[0..n) into sh.hist (and, for order-1 dense mode, the pair
+// counts (context row, symbol column) into sh.fc).
+__device__ void hist0(EncShared &sh, const u8 *in, u64 n)
+{
+    const u32 l = lane_id();
+    for (u32 s = l; s < 256; s += 64) sh.hist[s] = 0;
+    __syncthreads();
+    const u64 nd = (n + 3) / 4;
+    for (u64 t = l; t < nd; t += 64) {
+        const u32 w = load4(in + 4 * t, in + n);
+        const u32 nb = (4 * t + 4 <= n) ? 4 : (u32)(n - 4 * t);
+        for (u32 b = 0; b < nb; b++) atomicAdd(&sh.hist[(w >> (8 * b)) & 0xff], 1u);
+    }
+    __syncthreads();
+}
+
+// Pair counts: row(ctx) x col(sym) for every position (ctx of 0 is 0).
+template <bool DENSE>
+__device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts)
+{
+    const u32 l = lane_id();
+    const u64 nd = (n + 3) / 4;
+    for (u64 t = l; t < nd; t += 64) {
+        const u32 w = load4(in + 4 * t, in + n);
+        u32 prev = t > 0 ? in[4 * t - 1] : 0u;
+        const u32 nb = (4 * t + 4 <= n) ? 4 : (u32)(n - 4 * t);
+        for (u32 b = 0; b < nb; b++) {
+            const u32 s = (w >> (8 * b)) & 0xff;
+            if (DENSE)
+                atomicAdd(&sh.fc[(u32)sh.rowmap[prev] * DENSE_COLS + sh.colmap[s]], 1u);
+            else
+                atomicAdd(&gcounts[prev * 256 + s], 1u);
+            prev = s;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, u64 &out_len,
+                          u8 *ws)
+{
+    const u32 l = lane_id();
+    const u8 *in = src + jb.src_off;
+    u8 *o = dst + jb.dst_off;
+    const u64 n = jb.src_len;
+    const int order = jb.order;
+    if (order != 0 && order != 1) return FQXV_RANS_ERR_INVALID_ARG;
+    if (n >= (1ull << 32)) return FQXV_RANS_ERR_INVALID_ARG;
+    const u64 k_max = n < 256 ? n : 256;
+    const u64 bound = n == 0 ? 9 : (order == 0 ? O0_HDR : HDR0 + BITMAP + 512 * k_max + 4 * NSTATES) + 2 * n + 16;
+    if (jb.dst_cap < bound) return FQXV_RANS_ERR_DST_TOO_SMALL;
+    if (l < HDR0) o[l] = l == 0 ? (u8)order : (u8)(n >> (8 * (l - 1)));
+    if (n == 0) {
+        out_len = 9;
+        return 0;
+    }
+    u8 *wend = o + jb.dst_cap;
+    hist0(sh, in, n);
+
+    if (order == 0) {
+        u32 c[4], f[4];
+        for (int i = 0; i < 4; i++) c[i] = sh.hist[4 * l + i];
+        normalize_wave(c, f);
+        put_freqs(o + HDR0, f);
+        const u32 tot = f[0] + f[1] + f[2] + f[3];
+        u32 cum = wave_scan_incl(tot) - tot;
+        for (int i = 0; i < 4; i++) {
+            sh.fc[4 * l + i] = (f[i] << 12) | cum;
+            sh.mg[4 * l + i] = magic_of(f[i]);
+            cum += f[i];
+        }
+        __syncthreads();
+        TabRef tr{sh.fc, sh.mg, sh.rowmap, sh.colmap, 0};
+        u32 x;
+        const u64 W = walk(sh, in, n, tr, wend, x);
+        finish(o, O0_HDR, wend, W, x);
+        out_len = O0_HDR + 2 * W;
+        return 0;
+    }
+
+    // Order-1.  Alphabet A = symbols present; contexts are {0} u A.
+    u32 pres = 0;
+    for (int i = 0; i < 4; i++) pres += sh.hist[4 * l + i] != 0;
+    const u32 asz = uni(wave_sum(pres));
+    const bool dense = asz <= DENSE_COLS;
+    u32 *gcounts = (u32 *)ws;
+    u32 *gfc = (u32 *)(ws + 256ull * 256 * 4);
+    u32 *gmg = (u32 *)(ws + 2 * 256ull * 256 * 4);
+    if (dense) {
+        const u32 excl = wave_scan_incl(pres) - pres;
+        u32 d = excl;
+        for (int i = 0; i < 4; i++) {
+            const u32 s = 4 * l + i;
+            const bool p = sh.hist[s] != 0;
+            sh.colmap[s] = p ? (u8)d : (u8)0;
+            sh.rowmap[s] = p ? (u8)d : (u8)(DENSE_ROWS - 1);
+            d += p;
+        }
+        for (u32 e = l; e < TAB_ENTRIES; e += 64) sh.fc[e] = 0;
+        __syncthreads();
+        hist1<true>(sh, in, n, nullptr);
+    } else {
+        for (u64 e = l; e < 256 * 256; e += 64) gcounts[e] = 0;
+        __threadfence();
+        hist1<false>(sh, in, n, gcounts);
+        __threadfence();
+    }
+    // Present contexts (row sum > 0), ascending.
+    u32 pbits = 0;
+    for (int i = 0; i < 4; i++) {
+        const u32 ctx = 4 * l + i;
+        u32 any = 0;
+        if (dense) {
+            const u32 row = sh.rowmap[ctx];
+            const bool valid = sh.hist[ctx] != 0 || ctx == 0;
+            if (valid)
+                for (u32 cI = 0; cI < DENSE_COLS; cI++) any |= sh.fc[row * DENSE_COLS + cI];
+        } else {
+            for (u32 sI = 0; sI < 256; sI++) any |= gcounts[ctx * 256 + sI];
+        }
+        pbits |= (any != 0) << i;
+    }
+    const u32 pc = __popc(pbits);
+    const u32 pex = wave_scan_incl(pc) - pc;
+    const u32 K = uni(wave_sum(pc));
+    {
+        u32 d = pex;
+        for (int i = 0; i < 4; i++)
+            if ((pbits >> i) & 1) sh.ctx_of[d++] = (u8)(4 * l + i);
+    }
+    // Bitmap bytes: lane l holds the nibble for contexts 4l..4l+3.
+    {
+        const u32 other = __shfl_xor(pbits, 1);
+        if ((l & 1) == 0) o[HDR0 + l / 2] = (u8)(pbits | (other << 4));
+    }
+    __syncthreads();
+    const u64 hdr = HDR0 + BITMAP + 512ull * K + 4 * NSTATES;
+    for (u32 d = 0; d < K; d++) {
+        const u32 ctx = sh.ctx_of[d];
+        u32 c[4], f[4];
+        for (int i = 0; i < 4; i++) {
+            const u32 s = 4 * l + i;
+            if (dense)
+                c[i] = sh.hist[s] ? sh.fc[(u32)sh.rowmap[ctx] * DENSE_COLS + sh.colmap[s]] : 0u;
+            else
+                c[i] = gcounts[ctx * 256 + s];
+        }
+        normalize_wave(c, f);
+        put_freqs(o + HDR0 + BITMAP + 512ull * d, f);
+        const u32 tot = f[0] + f[1] + f[2] + f[3];
+        u32 cum = wave_scan_incl(tot) - tot;
+        __syncthreads(); // all lanes read this row's counts before it is overwritten
+        for (int i = 0; i < 4; i++) {
+            const u32 s = 4 * l + i;
+            if (dense) {
+                if (sh.hist[s]) {
+                    const u32 e = (u32)sh.rowmap[ctx] * DENSE_COLS + sh.colmap[s];
+                    sh.fc[e] = (f[i] << 12) | cum;
+                    sh.mg[e] = magic_of(f[i]);
+                }
+            } else {
+                gfc[ctx * 256 + s] = (f[i] << 12) | cum;
+                gmg[ctx * 256 + s] = magic_of(f[i]);
+            }
+            cum += f[i];
+        }
+        __syncthreads();
+    }
+    if (!dense) __threadfence();
+    TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u}
+                      : TabRef{gfc, gmg, sh.rowmap, sh.colmap, 2u};
+    u32 x;
+    const u64 W = walk(sh, in, n, tr, wend, x);
+    finish(o, hdr, wend, W, x);
+    out_len = hdr + 2 * W;
+    return 0;
+}
+
+} // namespace
+
+__global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, u8 *__restrict__ dst,
+                                                    const fqxv_rans_job *__restrict__ jobs, u32 count,
+                                                    const u32 *__restrict__ order, u32 *counter,
+                                                    i32 *status, u64 *out_len, u8 *ws)
+{
+    __shared__ EncShared sh;
+    u8 *myws = ws + (u64)blockIdx.x * WS_PER_WAVE;
+    for (;;) {
+        u32 w = 0;
+        if (lane_id() == 0) w = atomicAdd(counter, 1u);
+        w = uni(__shfl(w, 0));
+        if (w >= count) break;
+        const u32 j = order ? order[w] : w;
+        const fqxv_rans_job jb = jobs[j];
+        u64 ol = 0;
+        const int err = encode_job(sh, src, dst, jb, ol, myws);
+        if (lane_id() == 0) {
+            status[j] = err;
+            out_len[j] = err ? 0 : ol;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void k_rans_encode_one(const u8 *__restrict__ src, u8 *__restrict__ dst,
+                                                        fqxv_rans_job jb, i32 *status, u64 *out_len, u8 *ws)
+{
+    __shared__ EncShared sh;
+    u64 ol = 0;
+    const int err = encode_job(sh, src, dst, jb, ol, ws);
+    if (lane_id() == 0) {
+        *status = err;
+        *out_len = err ? 0 : ol;
+    }
+}
+
+size_t fqxv_enc_shared_bytes() { return sizeof(EncShared); }
+
+// ------------------------------------------------------------ host launchers
+__global__ void k_reset_counter_enc(u32 *counter) { *counter = 0; }
+
+hipError_t fqxv_launch_encode_batch(const u8 *src, u8 *dst, const fqxv_rans_job *jobs, u32 count,
+                                    const u32 *order, u32 *counter, i32 *status, u64 *out_len, u8 *ws,
+                                    u32 grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reset_counter_enc, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL(k_rans_encode, dim3(grid), dim3(64), 0, st, src, dst, jobs, count, order, counter,
+                       status, out_len, ws);
+    return hipGetLastError();
+}
+
+hipError_t fqxv_launch_encode_one(const u8 *src, u8 *dst, const fqxv_rans_job &jb, i32 *status,
+                                  u64 *out_len, u8 *ws, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rans_encode_one, dim3(1), dim3(64), 0, st, src, dst, jb, status, out_len, ws);
+    return hipGetLastError();
+}
+
+hipError_t fqxv_encode_occupancy(int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_rans_encode, 64, 0);
+}

@@ -1,0 +1,316 @@
+"""Host-side mirror of the fqxv-rans API over the gfx950 library.
+
+Reference surface (crates/fqxv-rans/src/lib.rs):
+
+    pub enum Order { Zero, One }                          lib.rs:58-64
+    pub enum Backend { Scalar, Sse42, Avx2, Avx512 }      lib.rs:67-97
+    pub enum Error { InputTooLarge, Malformed, NotImplemented }  lib.rs:40-52
+    pub fn encode(src, order) -> Result<Vec<u8>>          lib.rs:111-127
+    pub fn decode(src) -> Result<Vec<u8>>                 lib.rs:186-200
+    pub fn decode_bounded(src, max_len) -> Result<...>    lib.rs:226-231
+    pub fn decode_with(src, backend) -> Result<...>       lib.rs:241-255
+
+Same names, argument meaning and error behaviour: every malformed stream
+raises :class:`Malformed` carrying the reference's message; nothing here
+falls back to a CPU coder.  On top of that, :class:`Coder` exposes the
+device-resident batched forms (one persistent launch over many blocks, or
+one launch per block) used by the benchmark and the serving loop.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+# Declared lengths above this are treated like a failed `try_reserve_exact`
+# (model.rs:199-205): the decode still validates tables and states first.
+ALLOC_LIMIT = 1 << 36
+
+
+class Order(enum.IntEnum):
+    """lib.rs:58-64 -- the stream's order tag byte."""
+    Zero = 0
+    One = 1
+
+
+class Backend(enum.Enum):
+    """Which device form performs the coding (lib.rs:67-97 analogue).
+
+    ``Gfx950`` is the single-block launch, ``Gfx950Batched`` routes the call
+    through the persistent batched kernel.  Output is byte-identical.
+    """
+    Gfx950 = "gfx950"
+    Gfx950Batched = "gfx950-batched"
+
+    @staticmethod
+    def detect() -> "Backend":
+        return Backend.Gfx950
+
+
+class Error(Exception):
+    """lib.rs:40-52."""
+
+
+class InputTooLarge(Error):
+    pass
+
+
+class Malformed(Error):
+    """`Error::Malformed(&'static str)`; ``str(e)`` is the reference message."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+class NotImplemented_(Error):
+    pass
+
+
+class DeviceError(Error):
+    """The HIP runtime failed (no device, out of device memory, ...)."""
+
+
+def _raise(code: int):
+    if code == _lib.OK:
+        return
+    msg = _lib.strerror(code)
+    if code in (_lib.ERR_DEVICE,):
+        raise DeviceError(msg)
+    if code in (_lib.ERR_INVALID_ARG, _lib.ERR_DST_TOO_SMALL):
+        raise ValueError(msg)
+    raise Malformed(msg, code)
+
+
+def _as_bytes(src) -> bytes:
+    if isinstance(src, (bytes, bytearray, memoryview)):
+        return bytes(src)
+    if isinstance(src, np.ndarray):
+        return np.ascontiguousarray(src, dtype=np.uint8).tobytes()
+    return bytes(src)
+
+
+def peek_output_len(src) -> int | None:
+    """lib.rs:207-210."""
+    b = _as_bytes(src)
+    if len(b) < 9:
+        return None
+    return int.from_bytes(b[1:9], "little")
+
+
+@dataclass
+class BatchResult:
+    dst: object          # torch uint8 device tensor holding the outputs
+    dst_off: np.ndarray  # per-job output offsets (host)
+    out_len: object      # torch int64 device tensor
+    status: object       # torch int32 device tensor
+
+
+class Coder:
+    """One context on one HIP device (the library serialises its calls)."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self._L.fqxv_rans_ctx_create(device, ctypes.byref(h))
+        if rc:
+            raise DeviceError(f"fqxv_rans_ctx_create({device}): {_lib.strerror(rc)}")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.fqxv_rans_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------ reference API (host)
+    def encode(self, src, order: Order | int) -> bytes:
+        b = _as_bytes(src)
+        order = int(order)
+        if order not in (0, 1):
+            raise ValueError("order must be Order.Zero or Order.One")
+        cap = int(self._L.fqxv_rans_encode_bound(len(b), order))
+        dst = ctypes.create_string_buffer(cap)
+        out = ctypes.c_size_t(0)
+        sbuf = ctypes.create_string_buffer(b, len(b)) if b else None
+        rc = self._L.fqxv_rans_encode(self._h, sbuf, len(b), order, dst, cap, ctypes.byref(out))
+        _raise(rc)
+        return dst.raw[: out.value]
+
+    def decode_bounded(self, src, max_len: int) -> bytes:
+        b = _as_bytes(src)
+        n = peek_output_len(b)
+        if n is not None and n > max_len:
+            raise Malformed(_lib.strerror(_lib.ERR_BOUND), _lib.ERR_BOUND)
+        cap = n if (n is not None and n <= ALLOC_LIMIT) else 0
+        dst = ctypes.create_string_buffer(max(cap, 1))
+        out = ctypes.c_size_t(0)
+        sbuf = ctypes.create_string_buffer(b, len(b)) if b else None
+        rc = self._L.fqxv_rans_decode_bounded(self._h, sbuf, len(b), max_len, dst, cap,
+                                              ctypes.byref(out))
+        _raise(rc)
+        return dst.raw[: out.value]
+
+    def decode(self, src) -> bytes:
+        return self.decode_bounded(src, (1 << 64) - 1)
+
+    # ------------------------------------------------ device-resident forms
+    @staticmethod
+    def _stream(stream):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        return ctypes.c_void_p(stream.cuda_stream)
+
+    @staticmethod
+    def jobs_array(src_off, src_len, dst_off, dst_cap, max_len=None, order=None) -> np.ndarray:
+        k = len(src_off)
+        a = np.zeros(k, dtype=np.dtype(_lib.JOB_DTYPE_FIELDS))
+        a["src_off"] = src_off
+        a["src_len"] = src_len
+        a["dst_off"] = dst_off
+        a["dst_cap"] = dst_cap
+        a["max_len"] = (np.uint64(2**64 - 1) if max_len is None else max_len)
+        a["order"] = 0 if order is None else order
+        return a
+
+    def _run_batch(self, fn, src, dst, jobs_np, order_np, stream):
+        import torch
+        dev = src.device
+        jobs = torch.from_numpy(jobs_np.view(np.uint8).copy()).to(dev)
+        k = len(jobs_np)
+        status = torch.full((k,), -1, dtype=torch.int32, device=dev)
+        out_len = torch.zeros((k,), dtype=torch.int64, device=dev)
+        ordt = None
+        if order_np is not None:
+            ordt = torch.from_numpy(np.ascontiguousarray(order_np, dtype=np.uint32).view(np.int32)).to(dev)
+        rc = fn(self._h, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                ctypes.c_void_p(jobs.data_ptr()), k,
+                ctypes.c_void_p(ordt.data_ptr()) if ordt is not None else None,
+                ctypes.c_void_p(status.data_ptr()), ctypes.c_void_p(out_len.data_ptr()),
+                self._stream(stream))
+        _raise(rc)
+        return status, out_len, (jobs, ordt)
+
+    def encode_bound(self, n: int, order: int) -> int:
+        return int(self._L.fqxv_rans_encode_bound(int(n), int(order)))
+
+    def lpt_order(self, sizes) -> np.ndarray:
+        s = np.ascontiguousarray(sizes, dtype=np.uint64)
+        o = np.empty(len(s), dtype=np.uint32)
+        self._L.fqxv_rans_lpt_order(ctypes.c_void_p(s.ctypes.data), len(s), ctypes.c_void_p(o.ctypes.data))
+        return o
+
+    def encode_batch(self, src, src_off, src_len, orders, lpt: bool = True, stream=None) -> BatchResult:
+        """Encode every block of a device arena in one persistent launch."""
+        import torch
+        src_off = np.asarray(src_off, dtype=np.uint64)
+        src_len = np.asarray(src_len, dtype=np.uint64)
+        orders = np.asarray(orders, dtype=np.int32)
+        caps = np.array([self.encode_bound(int(n), int(o)) for n, o in zip(src_len, orders)],
+                        dtype=np.uint64)
+        dst_off = np.zeros_like(caps)
+        if len(caps):
+            dst_off[1:] = np.cumsum(caps)[:-1]
+        dst = torch.empty(int(caps.sum()) + 64, dtype=torch.uint8, device=src.device)
+        jobs = self.jobs_array(src_off, src_len, dst_off, caps, order=orders)
+        order_np = self.lpt_order(src_len) if lpt else None
+        status, out_len, keep = self._run_batch(self._L.fqxv_rans_encode_batch, src, dst, jobs, order_np, stream)
+        self._keep = keep
+        return BatchResult(dst, dst_off, out_len, status)
+
+    def decode_batch(self, src, src_off, src_len, out_len, max_len=None, lpt: bool = True,
+                     stream=None, dst=None, dst_off=None) -> BatchResult:
+        """Decode every stream of a device arena in one persistent launch.
+
+        ``out_len`` are the output capacities (normally each stream's n)."""
+        import torch
+        src_off = np.asarray(src_off, dtype=np.uint64)
+        src_len = np.asarray(src_len, dtype=np.uint64)
+        caps = np.asarray(out_len, dtype=np.uint64)
+        if dst_off is None:
+            dst_off = np.zeros_like(caps)
+            if len(caps):
+                dst_off[1:] = np.cumsum(caps)[:-1]
+        if dst is None:
+            dst = torch.empty(int(caps.sum()) + 64, dtype=torch.uint8, device=src.device)
+        jobs = self.jobs_array(src_off, src_len, dst_off, caps, max_len=max_len)
+        order_np = self.lpt_order(caps) if lpt else None
+        status, ol, keep = self._run_batch(self._L.fqxv_rans_decode_batch, src, dst, jobs, order_np, stream)
+        self._keep = keep
+        return BatchResult(dst, dst_off, ol, status)
+
+    def _one(self, fn, src, dst, job: _lib.Job, status, out_len, stream):
+        rc = fn(self._h, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                ctypes.byref(job), ctypes.c_void_p(status.data_ptr()),
+                ctypes.c_void_p(out_len.data_ptr()), self._stream(stream))
+        _raise(rc)
+
+    def decode_one_device(self, src, dst, job: _lib.Job, status, out_len, stream=None):
+        """The plain form: one kernel launch for one block (device pointers)."""
+        self._one(self._L.fqxv_rans_decode_one_device, src, dst, job, status, out_len, stream)
+
+    def encode_one_device(self, src, dst, job: _lib.Job, status, out_len, stream=None):
+        self._one(self._L.fqxv_rans_encode_one_device, src, dst, job, status, out_len, stream)
+
+    def max_waves(self) -> int:
+        w = ctypes.c_uint32(0)
+        _raise(self._L.fqxv_rans_max_waves(self._h, ctypes.byref(w)))
+        return int(w.value)
+
+
+_tls = threading.local()
+
+
+def default_coder() -> Coder:
+    c = getattr(_tls, "coder", None)
+    if c is None:
+        c = Coder(0)
+        _tls.coder = c
+    return c
+
+
+def encode(src, order: Order | int) -> bytes:
+    """lib.rs:111-127."""
+    return default_coder().encode(src, order)
+
+
+def decode(src) -> bytes:
+    """lib.rs:186-200."""
+    return default_coder().decode(src)
+
+
+def decode_bounded(src, max_len: int) -> bytes:
+    """lib.rs:226-231."""
+    return default_coder().decode_bounded(src, max_len)
+
+
+def decode_with(src, backend: Backend) -> bytes:
+    """lib.rs:241-255: force a device form; output is identical."""
+    if backend == Backend.Gfx950:
+        return default_coder().decode(src)
+    import torch
+    b = _as_bytes(src)
+    n = peek_output_len(b)
+    if n is None:
+        raise Malformed(_lib.strerror(_lib.ERR_TRUNCATED), _lib.ERR_TRUNCATED)
+    c = default_coder()
+    dev = torch.device("cuda", c.device)
+    s = torch.frombuffer(bytearray(b + b"\0" * 8), dtype=torch.uint8).to(dev)
+    cap = n if n <= ALLOC_LIMIT else 0
+    r = c.decode_batch(s, [0], [len(b)], [cap])
+    torch.cuda.synchronize(dev)
+    st = int(r.status[0])
+    _raise(st)
+    return r.dst[: int(r.out_len[0])].cpu().numpy().tobytes()

@@ -1,0 +1,143 @@
+/*
+ * fqxv_rans_hip.h -- C ABI of the MI355X (gfx950) rANS Nx16 coder.
+ *
+ * Drop-in boundary for the fqxv-rans crate (crates/fqxv-rans/src/lib.rs):
+ * the same wire format (scalar.rs:8-20), the same order-0/order-1 models,
+ * byte-identical streams.  Plain pointers and sizes only; no HIP or torch
+ * types appear in the signatures (a `void *stream` is a hipStream_t).
+ *
+ * Two families of entry points:
+ *   - host-buffer calls that mirror the Rust API one to one
+ *     (fqxv_rans_encode / _decode / _decode_bounded);
+ *   - device-resident batched calls over an arena of blocks
+ *     (fqxv_rans_encode_batch / _decode_batch), the form the container's
+ *     per-block dispatch (crates/fqxv/src/container/compress.rs:866-883,
+ *     decompress.rs:45-52) hands to one GPU.
+ *
+ * Return codes: FQXV_RANS_OK or one of the FQXV_RANS_ERR_* values below.
+ * Codes 1..7 are the reference's `Error::Malformed(&'static str)` messages
+ * (fqxv_rans_strerror gives the exact string); the Rust side maps every one
+ * of them to `Error::Malformed`.
+ *
+ * Thread safety: a context serialises its own host-buffer calls with an
+ * internal lock.  Batched calls on one context must not overlap on the
+ * device (they share the context's work-queue counters and workspace): use
+ * one context per concurrently used stream.
+ */
+#ifndef FQXV_RANS_HIP_H
+#define FQXV_RANS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FQXV_RANS_OK 0
+#define FQXV_RANS_ERR_TRUNCATED 1        /* "truncated stream"                        scalar.rs:256-263 */
+#define FQXV_RANS_ERR_UNKNOWN_ORDER 2    /* "unknown order tag"                       scalar.rs:153     */
+#define FQXV_RANS_ERR_FREQ_EXCEEDS 3     /* "rANS freq table exceeds TOTFREQ"         model.rs:165      */
+#define FQXV_RANS_ERR_FREQ_SUM 4         /* "rANS freq table does not sum to TOTFREQ" model.rs:169      */
+#define FQXV_RANS_ERR_OUTPUT_ALLOC 5     /* "rANS output length too large to allocate" model.rs:202     */
+#define FQXV_RANS_ERR_ABSENT_CONTEXT 6   /* "symbol references absent context"        scalar.rs:206     */
+#define FQXV_RANS_ERR_BOUND 7            /* "rANS output length exceeds caller bound" lib.rs:228        */
+#define FQXV_RANS_ERR_DST_TOO_SMALL 8    /* encode: destination smaller than fqxv_rans_encode_bound   */
+#define FQXV_RANS_ERR_INVALID_ARG 9      /* bad order / NULL pointer / bad job                         */
+#define FQXV_RANS_ERR_DEVICE 10          /* HIP runtime failure (allocation, launch, copy)             */
+
+/* Order tags (lib.rs:58-64; the stream's first byte, scalar.rs:25-26). */
+#define FQXV_RANS_ORDER0 0
+#define FQXV_RANS_ORDER1 1
+
+typedef struct fqxv_rans_ctx fqxv_rans_ctx;
+
+/* One block of a batched call.  Offsets index the caller's device arenas.
+ *   decode: src = one rANS stream, dst = its output region; `max_len` is the
+ *           decode_bounded bound (lib.rs:226-231; UINT64_MAX for plain
+ *           decode), `dst_cap` plays the fallible allocation of
+ *           model.rs:199-205; `order` is ignored (the stream carries it).
+ *   encode: src = raw bytes, dst = output region of at least
+ *           fqxv_rans_encode_bound(src_len, order) bytes; `max_len` ignored.
+ */
+typedef struct fqxv_rans_job {
+    uint64_t src_off;
+    uint64_t src_len;
+    uint64_t dst_off;
+    uint64_t dst_cap;
+    uint64_t max_len;
+    int32_t order;
+    uint32_t flags; /* reserved, must be 0 */
+} fqxv_rans_job;
+
+/* Exact message of a return code (the reference's Malformed strings). */
+const char *fqxv_rans_strerror(int code);
+
+/* Output capacity that always suffices for encode: header + 2n + 16 bytes of
+ * slack (n = 0 -> 9).  Replaces the Vec growth inside scalar::encode_order0/1
+ * (scalar.rs:29-111). */
+size_t fqxv_rans_encode_bound(size_t n, int order);
+
+/* lib.rs:207-210 `peek_output_len`: the declared n, or FQXV_RANS_ERR_TRUNCATED
+ * when fewer than 9 bytes are given. */
+int fqxv_rans_peek_len(const uint8_t *src, size_t len, uint64_t *n);
+
+/* Context on HIP device `device` (the library owns a stream and a lazily
+ * grown device workspace).  lib.rs has no context: the Rust wrapper keeps one
+ * per thread. */
+int fqxv_rans_ctx_create(int device, fqxv_rans_ctx **out);
+void fqxv_rans_ctx_destroy(fqxv_rans_ctx *ctx);
+
+/* lib.rs:111-127 `encode(src, order)`.  Host buffers.  Writes at most `cap`
+ * bytes; *out_len = stream length.  cap >= fqxv_rans_encode_bound() always
+ * suffices; a smaller cap fails with FQXV_RANS_ERR_DST_TOO_SMALL when the
+ * stream does not fit. */
+int fqxv_rans_encode(fqxv_rans_ctx *ctx, const uint8_t *src, size_t n, int order, uint8_t *dst,
+                     size_t cap, size_t *out_len);
+
+/* lib.rs:186-200 `decode(src)`.  Host buffers.  A declared n above `cap`
+ * fails with FQXV_RANS_ERR_OUTPUT_ALLOC (the reference's fallible
+ * allocation).  On any error *out_len = 0 and dst holds no partial output
+ * guarantee. */
+int fqxv_rans_decode(fqxv_rans_ctx *ctx, const uint8_t *src, size_t len, uint8_t *dst, size_t cap,
+                     size_t *out_len);
+
+/* lib.rs:226-231 `decode_bounded(src, max_len)`: FQXV_RANS_ERR_BOUND before
+ * any device work when the declared n exceeds max_len. */
+int fqxv_rans_decode_bounded(fqxv_rans_ctx *ctx, const uint8_t *src, size_t len, size_t max_len,
+                             uint8_t *dst, size_t cap, size_t *out_len);
+
+/* Batched device-resident encode/decode: one persistent-kernel launch codes
+ * all `count` jobs.  All pointers are device pointers (d_jobs included).
+ * `d_order` (optional, may be NULL) is a permutation of 0..count-1 giving the
+ * order in which waves take jobs (largest first is best; see
+ * fqxv_rans_lpt_order).  Per job: d_status[i] = return code,
+ * d_out_len[i] = bytes written.  Asynchronous on `stream` (NULL = the
+ * context's stream); the call is capturable into a hipGraph. */
+int fqxv_rans_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_t *d_dst,
+                           const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
+                           int32_t *d_status, uint64_t *d_out_len, void *stream);
+int fqxv_rans_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_t *d_dst,
+                           const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
+                           int32_t *d_status, uint64_t *d_out_len, void *stream);
+
+/* The plain form: one kernel launch per block (the reference's per-call
+ * granularity, lib.rs:111-200), device-resident, asynchronous on `stream`.
+ * `job` is a host struct; status and out_len are device pointers. */
+int fqxv_rans_decode_one_device(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_t *d_dst,
+                                const fqxv_rans_job *job, int32_t *d_status, uint64_t *d_out_len,
+                                void *stream);
+int fqxv_rans_encode_one_device(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_t *d_dst,
+                                const fqxv_rans_job *job, int32_t *d_status, uint64_t *d_out_len,
+                                void *stream);
+
+/* Host helper: job order, largest `sizes[i]` first (ties by index). */
+void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);
+
+/* Persistent-grid size used by the batched calls (workgroups of one wave). */
+int fqxv_rans_max_waves(fqxv_rans_ctx *ctx, uint32_t *waves);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""ctypes front-end to the C oracle (oracle/rans_oracle.c) -- TEST INFRASTRUCTURE.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this module, and only as the checker / CPU baseline.  The product
+package ``fqxv_amd`` never imports it.
+
+The functions mirror the reference API in crates/fqxv-rans/src/lib.rs:
+``encode`` (lib.rs:111-127), ``decode`` (lib.rs:186-200), ``decode_bounded``
+(lib.rs:226-231), plus ``normalize`` (model.rs:213-264) and the ``EncSym``
+map (model.rs:70-133) for the model-level tests.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "librans_oracle.so")
+
+OK = 0
+ERR_TRUNCATED = 1
+ERR_UNKNOWN_ORDER = 2
+ERR_FREQ_EXCEEDS = 3
+ERR_FREQ_SUM = 4
+ERR_OUTPUT_ALLOC = 5
+ERR_ABSENT_CONTEXT = 6
+ERR_BOUND = 7
+ERR_DST_TOO_SMALL = 8
+
+MESSAGES = {
+    ERR_TRUNCATED: "truncated stream",
+    ERR_UNKNOWN_ORDER: "unknown order tag",
+    ERR_FREQ_EXCEEDS: "rANS freq table exceeds TOTFREQ",
+    ERR_FREQ_SUM: "rANS freq table does not sum to TOTFREQ",
+    ERR_OUTPUT_ALLOC: "rANS output length too large to allocate",
+    ERR_ABSENT_CONTEXT: "symbol references absent context",
+    ERR_BOUND: "rANS output length exceeds caller bound",
+}
+
+
+class OracleError(Exception):
+    def __init__(self, code: int):
+        self.code = code
+        super().__init__(MESSAGES.get(code, f"oracle error {code}"))
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with the recipe in oracle/Makefile."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        c_sz = ctypes.c_size_t
+        p8 = ctypes.c_void_p
+        L.orc_encode_bound.restype = c_sz
+        L.orc_encode_bound.argtypes = [c_sz, ctypes.c_int]
+        L.orc_encode.argtypes = [p8, c_sz, ctypes.c_int, p8, c_sz, ctypes.POINTER(c_sz)]
+        L.orc_decode.argtypes = [p8, c_sz, p8, c_sz, ctypes.POINTER(c_sz), ctypes.POINTER(c_sz)]
+        L.orc_decode_bounded.argtypes = [p8, c_sz, c_sz, p8, c_sz, ctypes.POINTER(c_sz)]
+        L.orc_peek_len.argtypes = [p8, c_sz, ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_normalize.argtypes = [p8, p8]
+        L.orc_validate_freqs.argtypes = [p8]
+        L.orc_encsym_apply.restype = ctypes.c_uint32
+        L.orc_encsym_apply.argtypes = [p8, ctypes.c_uint32]
+        L.orc_encsym_new.argtypes = [ctypes.c_uint32, ctypes.c_uint32, p8]
+        for fn in (L.orc_encode_many, L.orc_decode_many):
+            fn.restype = ctypes.c_int
+        L.orc_encode_many.argtypes = [p8, p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
+        L.orc_decode_many.argtypes = [p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _as_u8(data) -> np.ndarray:
+    if isinstance(data, np.ndarray):
+        return np.ascontiguousarray(data, dtype=np.uint8)
+    return np.frombuffer(bytes(data), dtype=np.uint8)
+
+
+def encode_bound(n: int, order: int) -> int:
+    return int(lib().orc_encode_bound(n, order))
+
+
+def encode(data, order: int) -> bytes:
+    src = _as_u8(data)
+    cap = encode_bound(src.size, order)
+    dst = np.empty(cap, dtype=np.uint8)
+    out = ctypes.c_size_t(0)
+    rc = lib().orc_encode(_ptr(src) if src.size else None, src.size, order, _ptr(dst), cap,
+                          ctypes.byref(out))
+    if rc:
+        raise OracleError(rc)
+    return dst[: out.value].tobytes()
+
+
+def decode_ex(data, cap: int | None = None):
+    """Return (status, output bytes, consumed stream bytes)."""
+    src = _as_u8(data)
+    n = peek_len(src)
+    if cap is None:
+        cap = n if (n is not None and n <= (1 << 31)) else 0
+    dst = np.empty(max(cap, 1), dtype=np.uint8)
+    out = ctypes.c_size_t(0)
+    used = ctypes.c_size_t(0)
+    rc = lib().orc_decode(_ptr(src) if src.size else None, src.size, _ptr(dst), cap,
+                          ctypes.byref(out), ctypes.byref(used))
+    return rc, dst[: out.value].tobytes(), used.value
+
+
+def decode(data, cap: int | None = None) -> bytes:
+    rc, out, _ = decode_ex(data, cap)
+    if rc:
+        raise OracleError(rc)
+    return out
+
+
+def decode_bounded(data, max_len: int) -> bytes:
+    src = _as_u8(data)
+    n = peek_len(src)
+    if n is not None and n > max_len:
+        raise OracleError(ERR_BOUND)
+    return decode(src)
+
+
+def peek_len(data) -> int | None:
+    src = _as_u8(data)
+    v = ctypes.c_uint64(0)
+    rc = lib().orc_peek_len(_ptr(src) if src.size else None, src.size, ctypes.byref(v))
+    return None if rc else int(v.value)
+
+
+def normalize(counts) -> np.ndarray:
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    assert c.shape == (256,)
+    f = np.zeros(256, dtype=np.uint16)
+    lib().orc_normalize(_ptr(c), _ptr(f))
+    return f
+
+
+def validate_freqs(freq) -> int:
+    f = np.ascontiguousarray(freq, dtype=np.uint16)
+    return int(lib().orc_validate_freqs(_ptr(f)))
+
+
+class _EncSym(ctypes.Structure):
+    _fields_ = [("x_max", ctypes.c_uint64), ("rcp_freq", ctypes.c_uint32),
+                ("rcp_shift", ctypes.c_uint32), ("bias", ctypes.c_uint32),
+                ("cmpl_freq", ctypes.c_uint32), ("freq", ctypes.c_uint32),
+                ("cum", ctypes.c_uint32), ("use_div", ctypes.c_int)]
+
+
+def encsym(freq: int, cum: int) -> _EncSym:
+    e = _EncSym()
+    lib().orc_encsym_new(freq, cum, ctypes.byref(e))
+    return e
+
+
+def encsym_apply(e: _EncSym, v: int) -> int:
+    return int(lib().orc_encsym_apply(ctypes.byref(e), v))
+
+
+def encode_many(arena: np.ndarray, offs, lens, orders, threads: int):
+    """Encode many blocks over `threads` pthreads; returns (dst, dst_off, out_len, status)."""
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint64)
+    orders = np.ascontiguousarray(orders, dtype=np.int32)
+    caps = np.array([encode_bound(int(n), int(o)) for n, o in zip(lens, orders)], dtype=np.uint64)
+    doff = np.zeros_like(caps)
+    if caps.size:
+        doff[1:] = np.cumsum(caps)[:-1]
+    dst = np.empty(int(caps.sum()) + 1, dtype=np.uint8)
+    out_len = np.zeros(len(lens), dtype=np.uint64)
+    status = np.zeros(len(lens), dtype=np.int32)
+    lib().orc_encode_many(_ptr(arena), _ptr(offs), _ptr(lens), _ptr(orders), len(lens), _ptr(dst),
+                          _ptr(doff), _ptr(caps), _ptr(out_len), _ptr(status), threads)
+    return dst, doff, out_len, status
+
+
+def decode_many(arena: np.ndarray, offs, lens, out_caps, threads: int):
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint64)
+    caps = np.ascontiguousarray(out_caps, dtype=np.uint64)
+    doff = np.zeros_like(caps)
+    if caps.size:
+        doff[1:] = np.cumsum(caps)[:-1]
+    dst = np.empty(int(caps.sum()) + 1, dtype=np.uint8)
+    out_len = np.zeros(len(lens), dtype=np.uint64)
+    status = np.zeros(len(lens), dtype=np.int32)
+    lib().orc_decode_many(_ptr(arena), _ptr(offs), _ptr(lens), len(lens), _ptr(dst), _ptr(doff),
+                          _ptr(caps), _ptr(out_len), _ptr(status), threads)
+    return dst, doff, out_len, status

@@ -1,0 +1,77 @@
+"""The C-ABI library loads and exports what include/fqxv_rans_hip.h declares
+(no device calls: these run without a GPU)."""
+import ctypes
+import os
+import re
+import struct
+
+import pytest
+
+from conftest import ROOT
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "fqxv_rans_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fqxv_rans_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from fqxv_amd import _lib
+    return _lib.load()
+
+
+def test_exports_every_declared_symbol(L):
+    from fqxv_amd import _lib
+    declared = header_functions()
+    assert declared == sorted(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_job_struct_layout():
+    from fqxv_amd import _lib
+    import numpy as np
+    assert ctypes.sizeof(_lib.Job) == 48
+    assert np.dtype(_lib.JOB_DTYPE_FIELDS).itemsize == 48
+
+
+def test_strerror_matches_reference_messages(L, oracle):
+    from fqxv_amd import _lib
+    for code, msg in oracle.MESSAGES.items():
+        assert _lib.strerror(code) == msg
+
+
+def test_encode_bound(L, oracle):
+    for n in (0, 1, 31, 32, 1000, 1 << 20):
+        for order in (0, 1):
+            b = L.fqxv_rans_encode_bound(n, order)
+            assert b >= oracle.encode_bound(n, order)
+            assert b == (9 if n == 0 else oracle.encode_bound(n, order) + 16)
+
+
+def test_peek_len(L):
+    v = ctypes.c_uint64(0)
+    buf = ctypes.create_string_buffer(b"\x01" + struct.pack("<Q", 12345), 9)
+    assert L.fqxv_rans_peek_len(buf, 9, ctypes.byref(v)) == 0 and v.value == 12345
+    assert L.fqxv_rans_peek_len(buf, 8, ctypes.byref(v)) == 1
+
+
+def test_lpt_order(L):
+    import numpy as np
+    from fqxv_amd.rans import Coder
+    s = np.array([5, 9, 1, 9, 3], dtype=np.uint64)
+    o = np.empty(5, dtype=np.uint32)
+    L.fqxv_rans_lpt_order(ctypes.c_void_p(s.ctypes.data), 5, ctypes.c_void_p(o.ctypes.data))
+    assert list(o) == [1, 3, 0, 4, 2]
+
+
+def test_product_never_imports_oracle():
+    """The shipped package must not reach the checker."""
+    pkg = os.path.join(ROOT, "fqxv_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h", ".c")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in text.lower().replace("the oracle", ""), f

@@ -1,0 +1,188 @@
+"""GPU parity: the gfx950 library against the oracle, byte for byte.
+
+Every case runs the HIP kernels through the C ABI (libfqxv_rans_hip.so) and
+compares with the oracle on the same inputs: encoded streams must be
+identical, decoded bytes must be identical, and malformed streams must fail
+with the same error code.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import rng_bytes
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    cases = [b"", b"Q", b"AB", b"A" * 1000, bytes(range(256)) * 40]
+    cases += [bytes((i * 7 + 3) & 0xFF for i in range(n)) for n in (1, 2, 31, 32, 33, 63, 64, 65, 100)]
+    # lib.rs:439-447: dominant symbol (f > 2048), single symbol (f == 4096),
+    # two-symbol skew.
+    cases.append(b"A" * 60000 + bytes(ord("B") + i % 20 for i in range(4000)))
+    cases.append(b"Q" * 40000)
+    cases.append(bytes(ord("!") + (i % 3 == 0) * 7 for i in range(80000)))
+    cases.append(bytes((i * 131 % 251) for i in range(100003)))
+    for seed, (n, a) in enumerate([(5000, 40), (2000, 6), (100003, 256), (70001, 41), (4096, 2),
+                                   (1 << 16, 64), (1 << 16, 65), (30000, 200)]):
+        cases.append(rng_bytes(100 + seed, n, a))
+    return cases
+
+
+def test_host_api_matches_oracle(coder, oracle):
+    for data in _cases():
+        for order in (0, 1):
+            ref = oracle.encode(data, order)
+            got = coder.encode(data, order)
+            assert got == ref, (len(data), order)
+            assert coder.decode(ref) == data, (len(data), order)
+
+
+def test_fixture_streams_decode_and_reencode(coder, oracle, fixture_streams):
+    for e, s in fixture_streams:
+        out = coder.decode(s)
+        assert out == oracle.decode(s)
+        assert coder.encode(out, e["order"]) == s, e
+
+
+def test_normalize_residual_paths_on_device(coder, oracle):
+    """Inputs whose histograms take normalize's negative-residual, tie and
+    spill paths (hand-derived in test_oracle)."""
+    def with_counts(c):
+        return bytes(np.repeat(np.arange(256, dtype=np.uint8), c))
+    c1 = np.ones(256, np.int64)
+    c1[0] = 1_000_000
+    c2 = np.zeros(256, np.int64)
+    c2[5] = c2[9] = 100000
+    c2[[i for i in range(256) if i not in (5, 9)][:250]] = 1
+    c3 = np.ones(256, np.int64)
+    c3[:100] = 10000
+    for c in (c1, c2, c3):
+        data = with_counts(c)
+        for order in (0, 1):
+            assert coder.encode(data, order) == oracle.encode(data, order)
+
+
+def test_decode_bounded_and_errors(coder, oracle):
+    import fqxv_amd
+    enc = oracle.encode(b"A" * 64, 0)
+    bomb = bytearray(enc)
+    bomb[1:9] = struct.pack("<Q", 1 << 31)
+    with pytest.raises(fqxv_amd.Malformed) as ei:
+        coder.decode_bounded(bytes(bomb), 1 << 20)
+    assert str(ei.value) == "rANS output length exceeds caller bound"
+    assert coder.decode_bounded(enc, 1 << 20) == b"A" * 64
+    with pytest.raises(fqxv_amd.Malformed):
+        coder.decode_bounded(enc[:1], 1 << 20)
+    huge = b"\x00" + struct.pack("<Q", 2**64 - 1) + struct.pack("<H", 4096) + bytes(510) + bytes(128)
+    with pytest.raises(fqxv_amd.Malformed) as ei:
+        coder.decode(huge)
+    assert str(ei.value) == "rANS output length too large to allocate"
+
+
+def _err_code(fn):
+    import fqxv_amd
+    try:
+        fn()
+        return 0
+    except fqxv_amd.Malformed as e:
+        return e.code
+
+
+def test_malformed_streams_same_error_as_oracle(coder, oracle):
+    r = np.random.default_rng(77)
+    checked = 0
+    for it in range(400):
+        data = r.integers(0, 40, r.integers(0, 3000)).astype(np.uint8).tobytes()
+        enc = bytearray(oracle.encode(data, it & 1))
+        for _ in range(r.integers(0, 8)):
+            if enc:
+                enc[r.integers(0, len(enc))] = r.integers(0, 256)
+        if enc and r.random() < 0.5:
+            del enc[r.integers(0, len(enc)):]
+        b = bytes(enc)
+        n = oracle.peek_len(b)
+        if n is not None and n > max(len(data), 1) * 4:
+            continue
+        rc, out, _ = oracle.decode_ex(b)
+        got = _err_code(lambda: coder.decode(b))
+        assert got == rc, (it, rc, got)
+        if rc == 0:
+            assert coder.decode(b) == out
+        checked += 1
+    assert checked > 200
+
+
+def test_batched_equals_single_and_oracle(coder, oracle):
+    import torch
+    blocks = [c for c in _cases() if len(c) < 200000] * 2
+    orders = [i & 1 for i in range(len(blocks))]
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    offs = np.zeros(len(blocks), dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(blocks) + bytes(64), dtype=np.uint8).copy()).cuda()
+    r = coder.encode_batch(arena, offs, sizes, orders)
+    torch.cuda.synchronize()
+    st = r.status.cpu().numpy()
+    ol = r.out_len.cpu().numpy()
+    host = r.dst.cpu().numpy()
+    assert np.all(st == 0)
+    for j, b in enumerate(blocks):
+        got = host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes()
+        assert got == oracle.encode(b, orders[j]), j
+    d = coder.decode_batch(r.dst, r.dst_off, ol, sizes)
+    torch.cuda.synchronize()
+    assert np.all(d.status.cpu().numpy() == 0)
+    out = d.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        assert out[int(d.dst_off[j]):int(d.dst_off[j]) + len(b)].tobytes() == b, j
+
+
+def _corpus_roundtrip(coder, oracle, corpus, check_blocks):
+    """Batched encode on device, byte-identity vs oracle on `check_blocks`,
+    full device round trip."""
+    import torch
+    arena = torch.from_numpy(corpus.data).cuda()
+    r = coder.encode_batch(arena, corpus.offs, corpus.sizes, corpus.orders)
+    torch.cuda.synchronize()
+    st = r.status.cpu().numpy()
+    assert np.all(st == 0)
+    ol = r.out_len.cpu().numpy()
+    host = None
+    for j in check_blocks:
+        if host is None:
+            host = r.dst.cpu().numpy()
+        got = host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes()
+        assert got == oracle.encode(corpus.block(j), int(corpus.orders[j])), j
+    d = coder.decode_batch(r.dst, r.dst_off, ol, corpus.sizes)
+    torch.cuda.synchronize()
+    assert np.all(d.status.cpu().numpy() == 0)
+    assert np.array_equal(d.out_len.cpu().numpy(), corpus.sizes.astype(np.int64))
+    n = corpus.total
+    assert torch.equal(d.dst[:n], arena[:n])
+    return r, ol
+
+
+def test_workload_binned_1mib(coder, oracle):
+    from fqxv_amd import synth
+    c = synth.workload("binned_1mib")
+    _corpus_roundtrip(coder, oracle, c, [0])
+
+
+def test_workload_normal_64x1mib(coder, oracle):
+    from fqxv_amd import synth
+    c = synth.workload("normal_64x1mib")
+    _corpus_roundtrip(coder, oracle, c, range(0, 64, 4))
+
+
+def test_workload_markov_64x4mib(coder, oracle):
+    from fqxv_amd import synth
+    c = synth.workload("markov_64x4mib")
+    _corpus_roundtrip(coder, oracle, c, [0, 1, 63])
+
+
+def test_workload_mixed_prefix(coder, oracle):
+    from fqxv_amd import synth
+    c = synth.workload("mixed_10k", blocks=400)
+    _corpus_roundtrip(coder, oracle, c, range(0, 400, 7))

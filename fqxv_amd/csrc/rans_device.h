@@ -118,7 +118,7 @@ __device__ __forceinline__ u32 load4(const u8 *p, const u8 *end)
     if (ph == 0) {
         if (p + 4 <= end) return *base;
     } else if ((const u8 *)(base + 2) <= end) {
-        return __builtin_amdgcn_alignbyte(base[1], base[0], ph * 8);
+        return __builtin_amdgcn_alignbyte(base[1], base[0], ph); // shift in bytes
     }
     u32 v = 0;
     for (u32 i = 0; i < 4; i++)

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -183,6 +184,14 @@ int fqxv_rans_ctx_create(int device, fqxv_rans_ctx **out)
     }
     c->dec_waves = (u32)std::max(1, dec_occ) * (u32)prop.multiProcessorCount;
     c->enc_waves = (u32)std::max(1, enc_occ) * (u32)prop.multiProcessorCount;
+    // FQXV_RANS_GRID caps the persistent grid (tuning and tests).
+    if (const char *g = getenv("FQXV_RANS_GRID")) {
+        const long v = strtol(g, nullptr, 10);
+        if (v > 0) {
+            c->dec_waves = std::min<u32>(c->dec_waves, (u32)v);
+            c->enc_waves = std::min<u32>(c->enc_waves, (u32)v);
+        }
+    }
     *out = c;
     return FQXV_RANS_OK;
 }

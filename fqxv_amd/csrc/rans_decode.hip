@@ -11,81 +11,133 @@
 //     x' = f*(x>>12) + (slot-cum) (one v_mad_u32_u24; never overflows);
 //   * renormalisation words are consumed in lane order: a ballot marks the
 //     lanes with x' < 2^16, v_mbcnt gives each its rank, and lane l takes
-//     word[pos + rank] from an LDS ring that is refilled one 512-byte chunk
-//     at a time, three chunks ahead of the read position;
+//     word[pos + rank] from an LDS ring refilled one 512-byte chunk at a
+//     time.  The refill loads raw aligned dwords one chunk ahead (no waits,
+//     no branches) and realigns them with v_alignbyte when they are written
+//     to LDS at the next chunk boundary; the ring's first 32 words are
+//     mirrored past its end so a round's reads never wrap;
+//   * decoded bytes are staged in LDS and written 1 KiB (32 rounds) at a
+//     time with one 16-byte store per lane;
 //   * order-1: symbol i's context is symbol i-1 (scalar.rs:203), a serial
-//     chain through the round.  Every present context (K <= 64, dense id =
-//     lane) resolves its candidate symbol for all 32 slots in parallel, then
-//     the chain is 32 v_readlane steps through those candidate vectors.
+//     chain through the round.  Each present context (K <= 64) owns a lane;
+//     each round every lane fetches, for all 32 slots, the code of the symbol
+//     its context would decode there (u8 codes, slot-major table), and the
+//     chain is then 32 v_readlane steps through those candidate vectors, the
+//     code being the lane of the next context.  v_writelane hands each
+//     position its (context, symbol) pair and the state update is
+//     lane-parallel from a (context x symbol) table in LDS.
 #include "rans_device.h"
 
 namespace {
 
-constexpr u32 RING_CHUNK_WORDS = 256;            // words per ring chunk
+constexpr u32 RING_CHUNK_WORDS = 256;                // 512 bytes per chunk
 constexpr u32 RING_CHUNKS = 4;
 constexpr u32 RING_WORDS = RING_CHUNK_WORDS * RING_CHUNKS;
-constexpr u32 RING_DW = RING_WORDS / 2;
+constexpr u32 RING_MIRROR = 32;                      // words mirrored past the end
+constexpr u32 OUT_ROUNDS = 32;                       // rounds per output flush
+constexpr u32 OUT_BYTES = OUT_ROUNDS * NSTATES;      // 1 KiB
+constexpr u32 UMAX = 64;                             // order-1 fast path lane space
 
 struct DecShared {
-    u32 tab[TOTFREQ];     // order-0 packed slot table
-    u32 ring[RING_DW];    // renormalisation word ring
-    u32 fc[256];          // f << 16 | cum for the table being built
-    u16 dense[256];       // order-1: context byte -> dense id (0xffff absent)
+    u32 tab[TOTFREQ];                                 // o0 slot table; o1: fc2[64][64]
+    u32 ring[(RING_WORDS + RING_MIRROR) / 2];         // renormalisation word ring
+    union {
+        u32 outst[OUT_BYTES / 4];                     // output staging (decode loop)
+        u32 fc[256];                                  // f << 16 | cum (table builds)
+    };
+    u32 sbuf[NSTATES];                                // order-1: broadcast slots
+    u16 dense[256];                                   // order-1: context byte -> dense id
+    u8 ulane[256];                                    // order-1: symbol -> lane in U
 };
 
-// Words region reader: a ring of RING_CHUNKS chunks in LDS.  Chunk c covers
-// words [256c, 256c + 256) of the region.  When the read position enters
-// chunk c, chunks c, c+1, c+2 are resident; chunk c+3 is in flight in
-// registers.
+// Words region reader (see the file comment).  Chunk c covers words
+// [256c, 256c + 256).  When the read position enters chunk c, chunks c, c+1,
+// c+2 are resident and chunk c+3's raw dwords are in flight in r0..r3.
 struct WordRing {
-    const u8 *base;   // first byte of the words region
-    const u8 *end;    // one past the last stream byte
-    u64 chunk;        // chunk holding the read position
-    u32 pf0, pf1;     // prefetched dwords of chunk `chunk + 3`
+    const u32 *A;  // 4-byte-aligned address at or below the words region
+    u32 ph;        // byte offset of the region within A[0]
+    i64 last;      // index of the last dword of A holding a stream byte (>= -1)
+    u64 chunk;
+    u32 r0, r1, r2, r3;
 
-    __device__ void fetch(u64 c, u32 &d0, u32 &d1) const
+    __device__ void init(const u8 *wbase, const u8 *end)
     {
-        const u8 *p = base + c * (2 * RING_CHUNK_WORDS) + 4 * lane_id();
-        d0 = load4(p, end);
-        d1 = load4(p + 256, end);
+        ph = (u32)((uintptr_t)wbase & 3);
+        A = (const u32 *)(wbase - ph);
+        last = ((i64)(end - wbase) + (i64)ph - 1) >> 2;
     }
-    __device__ void put(u32 *ring, u64 c, u32 d0, u32 d1) const
+    // Raw loads for chunk c: dwords 128c + l (+1) and 128c + 64 + l (+1),
+    // clamped to the last dword that holds a stream byte.  Words past the end
+    // of the stream are never consumed (the caller checks the count).
+    __device__ __forceinline__ void fetch(u64 c)
     {
-        u32 *slot = ring + (u32)(c % RING_CHUNKS) * (RING_CHUNK_WORDS / 2);
-        slot[lane_id()] = d0;
-        slot[lane_id() + 64] = d1;
+        const i64 q = (i64)(128 * c) + (i64)lane_id();
+        r0 = A[q < last ? q : last];
+        r1 = A[q + 1 < last ? q + 1 : last];
+        r2 = A[q + 64 < last ? q + 64 : last];
+        r3 = A[q + 65 < last ? q + 65 : last];
+    }
+    __device__ __forceinline__ void put(u32 *ring, u64 c) const
+    {
+        const u32 l = lane_id();
+        const u32 v0 = __builtin_amdgcn_alignbyte(r1, r0, ph);
+        const u32 v1 = __builtin_amdgcn_alignbyte(r3, r2, ph);
+        const u32 slot = (u32)(c % RING_CHUNKS) * (RING_CHUNK_WORDS / 2);
+        ring[slot + l] = v0;
+        ring[slot + 64 + l] = v1;
+        if (slot == 0 && l < RING_MIRROR / 2) ring[RING_WORDS / 2 + l] = v0;
     }
     __device__ void reset(u32 *ring, u64 c0)
     {
         chunk = c0;
         for (u64 c = c0; c < c0 + 3; c++) {
-            u32 a, b;
-            fetch(c, a, b);
-            put(ring, c, a, b);
+            fetch(c);
+            put(ring, c);
         }
-        fetch(c0 + 3, pf0, pf1);
+        fetch(c0 + 3);
         __syncthreads();
     }
-    // Called after the read position advanced; at most one boundary per round.
     __device__ __forceinline__ void advance(u32 *ring, u64 pos)
     {
         if ((pos / RING_CHUNK_WORDS) != chunk) {
             chunk++;
-            put(ring, chunk + 2, pf0, pf1);
-            fetch(chunk + 3, pf0, pf1);
-            __syncthreads();
+            put(ring, chunk + 2);
+            fetch(chunk + 3);
         }
     }
-    __device__ __forceinline__ u32 word(const u32 *ring, u64 idx) const
+    // Word pos + rank, rank < 32.
+    __device__ __forceinline__ u32 word(const u32 *ring, u64 pos, u32 rank) const
     {
-        return ((const u16 *)ring)[idx % RING_WORDS];
+        return ((const u16 *)ring)[(u32)(pos % RING_WORDS) + rank];
     }
     __device__ u32 word_direct(u64 idx) const
     {
-        const u8 *p = base + 2 * idx;
+        const u8 *p = (const u8 *)A + ph + 2 * idx;
         return (u32)p[0] | ((u32)p[1] << 8);
     }
 };
+
+// Output staging: lane l < 32 writes its byte of round `row` into the stage;
+// every 32 rounds the stage is written out, 16 bytes per lane.
+__device__ __forceinline__ void stage_byte(DecShared &sh, u32 row, u32 sym)
+{
+    ((u8 *)sh.outst)[row * NSTATES + lane_id()] = (u8)sym;
+}
+
+__device__ __forceinline__ void flush(const DecShared &sh, u8 *dst, u32 nbytes)
+{
+    const u32 l = lane_id();
+    if (nbytes == OUT_BYTES && ((uintptr_t)dst & 15) == 0) {
+        const uint4 v = ((const uint4 *)sh.outst)[l];
+        ((uint4 *)dst)[l] = v;
+    } else {
+        const u8 *st = (const u8 *)sh.outst;
+        for (u32 i = 0; i < 16; i++) {
+            const u32 b = 16 * l + i;
+            if (b < nbytes) dst[b] = st[b];
+        }
+    }
+}
 
 // Read one frequency table (256 little-endian u16 at p), validate it
 // (model.rs:160-174), and leave f << 16 | cum in sh.fc.  Returns 0 or the
@@ -111,7 +163,7 @@ __device__ int load_freq_table(DecShared &sh, const u8 *p, const u8 *end)
     return 0;
 }
 
-// Owner symbol of every slot from sh.fc: lane l fills slots [64l, 64l+64).
+// Owner symbol of every slot from sh.fc: lane l visits slots [64l, 64l+64).
 // emit(slot, sym, f, cum) is called once per slot.
 template <typename Emit>
 __device__ __forceinline__ void for_each_slot(const DecShared &sh, Emit emit)
@@ -135,15 +187,15 @@ __device__ __forceinline__ void for_each_slot(const DecShared &sh, Emit emit)
     }
 }
 
-__device__ __forceinline__ void read_states(const u8 *p, const u8 *end, u32 &x)
+__device__ __forceinline__ u32 read_state(const u8 *p, const u8 *end)
 {
-    x = lane_id() < NSTATES ? load4(p + 4 * lane_id(), end) : 0u;
+    return lane_id() < NSTATES ? load4(p + 4 * lane_id(), end) : 0u;
 }
 
-// Exact scalar step for rounds in which some state would need more than one
-// word (only a malformed stream can get there: a valid state never reaches
-// v == 0, see step_state, scalar.rs:229-238).  Lanes are processed in
-// ascending order, each pulling words until v >= RANS_L.
+// Exact scalar step for a round in which some state would need more than
+// one word (only a malformed stream gets there: a valid state never reaches
+// v == 0, see step_state, scalar.rs:229-238).  Lanes go in ascending order,
+// each pulling words until v >= RANS_L.
 __device__ int slow_round(u32 &v, u32 act_mask_lo, u64 &pos, u64 avail, const WordRing &wr)
 {
     const u32 l = lane_id();
@@ -165,6 +217,29 @@ __device__ int slow_round(u32 &v, u32 act_mask_lo, u64 &pos, u64 avail, const Wo
     return err;
 }
 
+// Renormalise the active lanes' v and commit them to x.  Returns 0 or the
+// truncation error.  Words are taken in lane order (scalar.rs:171-176).
+__device__ __forceinline__ int renorm(DecShared &sh, WordRing &wr, u32 &x, u32 v, bool act, u64 &pos,
+                                      u64 avail)
+{
+    if (ballot(act && v == 0)) {
+        int err = slow_round(v, (u32)ballot(act), pos, avail, wr);
+        if (err) return err;
+        if (act) x = v;
+        wr.reset(sh.ring, pos / RING_CHUNK_WORDS);
+        return 0;
+    }
+    const bool need = act && v < RANS_L;
+    const u64 m = ballot(need);
+    const u32 k = (u32)__popcll(m);
+    if (pos + k > avail) return FQXV_RANS_ERR_TRUNCATED;
+    const u32 w = wr.word(sh.ring, pos, rank_below(m));
+    if (act) x = need ? ((v << 16) | w) : v;
+    pos += k;
+    wr.advance(sh.ring, pos);
+    return 0;
+}
+
 // ---------------------------------------------------------------- order-0
 __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap)
 {
@@ -174,8 +249,7 @@ __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     int err = load_freq_table(sh, s + HDR0, end);
     if (err) return err;
     if (len < O0_HDR) return FQXV_RANS_ERR_TRUNCATED;
-    u32 x;
-    read_states(s + HDR0 + 512, end, x);
+    u32 x = read_state(s + HDR0 + 512, end);
     if (n > cap) return FQXV_RANS_ERR_OUTPUT_ALLOC;
 
     for_each_slot(sh, [&](u32 k, u32 sym, u32 f, u32 cum) {
@@ -183,217 +257,239 @@ __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     });
 
     WordRing wr;
-    wr.base = s + O0_HDR;
-    wr.end = end;
+    wr.init(s + O0_HDR, end);
     const u64 avail = (len - O0_HDR) / 2;
     wr.reset(sh.ring, 0);
 
+    const bool lo = l < NSTATES;
     const u64 full = n / NSTATES;
     const u32 rem = (u32)(n % NSTATES);
-    const u64 rounds = full + (rem ? 1 : 0);
     u64 pos = 0;
-    u8 *o = out + l;
-    for (u64 r = 0; r < rounds; r++, o += NSTATES) {
-        const bool act = l < NSTATES && (r < full || l < rem);
+    u32 row = 0;
+    u8 *o = out;
+    for (u64 r = 0; r < full; r++) {
         const u32 p = sh.tab[x & (TOTFREQ - 1)];
-        const u32 f = (p >> 20) + 1;
-        u32 v = f * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
-        if (ballot(act && v == 0)) {
-            err = slow_round(v, (u32)ballot(act), pos, avail, wr);
-            if (err) return err;
-            if (act) {
-                x = v;
-                *o = (u8)p;
-            }
-            wr.reset(sh.ring, pos / RING_CHUNK_WORDS);
-            continue;
+        const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
+        if ((err = renorm(sh, wr, x, v, lo, pos, avail))) return err;
+        if (lo) stage_byte(sh, row, p);
+        if (++row == OUT_ROUNDS) {
+            flush(sh, o, OUT_BYTES);
+            o += OUT_BYTES;
+            row = 0;
         }
-        const bool need = act && v < RANS_L;
-        const u64 m = ballot(need);
-        const u32 k = (u32)__popcll(m);
-        if (pos + k > avail) return FQXV_RANS_ERR_TRUNCATED;
-        const u32 w = wr.word(sh.ring, pos + rank_below(m));
-        if (act) {
-            x = need ? ((v << 16) | w) : v;
-            *o = (u8)p;
-        }
-        pos += k;
-        wr.advance(sh.ring, pos);
     }
+    if (rem) {
+        const bool act = l < rem;
+        const u32 p = sh.tab[x & (TOTFREQ - 1)];
+        const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
+        if ((err = renorm(sh, wr, x, v, act, pos, avail))) return err;
+        if (act) stage_byte(sh, row, p);
+    }
+    flush(sh, o, row * NSTATES + rem);
     return 0;
 }
 
 // ---------------------------------------------------------------- order-1
-// Workspace layout per wave, for dense context ids d:
-//   codes[slot * KS + d] (u16) = sym | chain(sym) << 8, slot-major so that
-//     the 64 lanes resolving one slot for 64 contexts read one cache line;
-//     chain(sym) is sym's dense id when K <= 64, else 0xff (absent) / 0;
-//   fcw[d * 256 + sym] (u32) = f << 12 | cum.
-__device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap, u8 *ws)
+// Block setup shared by both order-1 paths: presence bitmap, then every
+// table read and validated in ascending context order (scalar.rs:181-194).
+// Leaves K, sh.dense, and the symbol presence of lane l's symbols in symp.
+__device__ int o1_validate(DecShared &sh, const u8 *s, const u8 *end, u32 &K, u32 &symp)
 {
     const u32 l = lane_id();
     const u64 len = (u64)(end - s);
     if (len < HDR0 + BITMAP) return FQXV_RANS_ERR_TRUNCATED;
-    // Presence bitmap (scalar.rs:181-194): lane l owns contexts 4l..4l+3.
     const u32 bm = s[HDR0 + l / 2];
     const u32 bits = (bm >> (4 * (l & 1))) & 0xf;
     const u32 cnt = __popc(bits);
     const u32 excl = wave_scan_incl(cnt) - cnt;
-    const u32 K = uni(wave_sum(cnt));
-    {
-        u32 d = excl;
-        for (int i = 0; i < 4; i++) {
-            const bool pr = (bits >> i) & 1;
-            sh.dense[4 * l + i] = pr ? (u16)d : (u16)0xffff;
-            d += pr;
-        }
+    K = uni(wave_sum(cnt));
+    u32 d = excl;
+    for (int i = 0; i < 4; i++) {
+        const bool pr = (bits >> i) & 1;
+        sh.dense[4 * l + i] = pr ? (u16)d : (u16)0xffff;
+        d += pr;
     }
     __syncthreads();
-    u16 *codes = (u16 *)ws;
-    u32 *fcw = (u32 *)(ws + WS_CODES);
-    const u32 KS = K <= 64 ? 64 : 256;
+    symp = 0;
     const u8 *t = s + HDR0 + BITMAP;
-    for (u32 d = 0; d < K; d++, t += 512) {
+    for (u32 c = 0; c < K; c++, t += 512) {
         if ((u64)(t + 512 - s) > len) return FQXV_RANS_ERR_TRUNCATED;
         int err = load_freq_table(sh, t, end);
         if (err) return err;
+        for (int i = 0; i < 4; i++) symp |= (u32)((sh.fc[4 * l + i] >> 16) != 0) << i;
+    }
+    return 0;
+}
+
+// Fast path: K <= 64 contexts and |contexts u symbols| <= 64.
+// Workspace: t8[slot * kp + d] (u8) = lane(sym) | 0x40 when sym is no context.
+// LDS: fc2[d * 64 + lane(sym)] = (f-1) << 20 | cum << 8 | sym.
+__device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
+                              u64 hdr, u8 *ws)
+{
+    const u32 l = lane_id();
+    const u32 kp = (K + 3) & ~3u;
+    u8 *t8 = ws;
+    u32 *fc2 = sh.tab;
+    const u8 *t = s + HDR0 + BITMAP;
+    for (u32 d = 0; d < K; d++, t += 512) {
+        (void)load_freq_table(sh, t, end); // validated already
+        for_each_slot(sh, [&](u32 k, u32 sym, u32 f, u32 cum) {
+            const u32 ul = sh.ulane[sym];
+            t8[(u64)k * kp + d] = (u8)(ul | (ul >= K ? 0x40u : 0u));
+            if (k == cum) fc2[d * UMAX + ul] = ((f - 1) << 20) | (cum << 8) | sym;
+        });
+        __syncthreads();
+    }
+    __threadfence();
+
+    WordRing wr;
+    wr.init(s + hdr, end);
+    const u64 avail = ((u64)(end - s) - hdr) / 2;
+    wr.reset(sh.ring, 0);
+
+    u32 prev = sh.dense[0];
+    if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT; // context 0 of symbol 0 (scalar.rs:203)
+    const bool lo = l < NSTATES;
+    const u64 rounds = (n + NSTATES - 1) / NSTATES;
+    u64 pos = 0;
+    u32 row = 0;
+    u8 *o = out;
+    const u8 *mycol = t8 + l;
+    for (u64 r = 0; r < rounds; r++) {
+        const u32 cntr = (r + 1) * NSTATES <= n ? NSTATES : (u32)(n - r * NSTATES);
+        const u32 slot = x & (TOTFREQ - 1);
+        if (lo) sh.sbuf[l] = slot;
+        u32 cand[NSTATES];
+#pragma unroll
+        for (u32 j = 0; j < NSTATES; j++) cand[j] = mycol[(u64)sh.sbuf[j] * kp];
+        u32 u = prev, cv = 0;
+#pragma unroll
+        for (u32 j = 0; j < NSTATES; j++) {
+            const u32 e = __builtin_amdgcn_readlane(cand[j], u);
+            cv = writelane(u | (e << 8), j, cv);
+            u = e & 0x3f;
+        }
+        prev = u;
+        // First position whose symbol is no context: the next symbol fails.
+        const u64 nb = ballot(lo && l < cntr && ((cv >> 8) & 0x40));
+        const u32 jb = nb ? (u32)__builtin_ctzll(nb) : NSTATES;
+        const bool act = lo && l < cntr && l <= jb;
+        const u32 ent = fc2[(cv & 0xff) * UMAX + ((cv >> 8) & 0x3f)];
+        const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
+        int err = renorm(sh, wr, x, v, act, pos, avail);
+        if (err) return err;
+        if (act) stage_byte(sh, row, ent);
+        if (jb < NSTATES && (u64)r * NSTATES + jb + 1 < n) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        if (++row == OUT_ROUNDS) {
+            // The last round may be partial: never write past out + n.
+            const u64 left = n - (u64)(o - out);
+            flush(sh, o, left < OUT_BYTES ? (u32)left : OUT_BYTES);
+            o += OUT_BYTES;
+            row = 0;
+        }
+    }
+    if (row) flush(sh, o, (u32)(n - (u64)(o - out)));
+    return 0;
+}
+
+// General path (K > 64 or a wide symbol set): serial chain through global
+// tables.  Correct but slow; no synthetic or fixture stream reaches it.
+// Workspace: sym16[slot * 256 + d] (u16 symbol), fcw[d * 256 + sym].
+__device__ int decode_o1_wide(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
+                              u64 hdr, u8 *ws)
+{
+    const u32 l = lane_id();
+    u16 *sym16 = (u16 *)ws;
+    u32 *fcw = (u32 *)(ws + WS_CODES);
+    const u8 *t = s + HDR0 + BITMAP;
+    for (u32 d = 0; d < K; d++, t += 512) {
+        (void)load_freq_table(sh, t, end);
         for (int i = 0; i < 4; i++) {
             const u32 e = sh.fc[4 * l + i];
             fcw[d * 256 + 4 * l + i] = ((e >> 16) << 12) | (e & 0xffff);
         }
-        u16 *cd = codes + d;
-        for_each_slot(sh, [&](u32 k, u32 sym, u32, u32) {
-            const u32 ds = sh.dense[sym];
-            const u32 chain = K <= 64 ? (ds < 64 ? ds : 0xffu) : 0u;
-            cd[(u64)k * KS] = (u16)(sym | (chain << 8));
-        });
+        for_each_slot(sh, [&](u32 k, u32 sym, u32, u32) { sym16[(u64)k * 256 + d] = (u16)sym; });
         __syncthreads();
     }
-    const u64 hdr = HDR0 + BITMAP + 512ull * K + 4 * NSTATES;
-    if (len < hdr) return FQXV_RANS_ERR_TRUNCATED;
-    u32 x;
-    read_states(t, end, x);
-    if (n > cap) return FQXV_RANS_ERR_OUTPUT_ALLOC;
-    // The tables were written through the vector memory path by all lanes;
-    // make them visible (and drop any stale L1 lines) before the reads.
     __threadfence();
 
     WordRing wr;
-    wr.base = s + hdr;
-    wr.end = end;
-    const u64 avail = (len - hdr) / 2;
+    wr.init(s + hdr, end);
+    const u64 avail = ((u64)(end - s) - hdr) / 2;
     wr.reset(sh.ring, 0);
 
-    // Context of symbol 0 is 0 (scalar.rs:203).
     u32 prev = sh.dense[0];
-    if (K <= 64 && prev == 0xffff) prev = 0xff;
-    const u64 full = n / NSTATES;
-    const u32 rem = (u32)(n % NSTATES);
-    const u64 rounds = full + (rem ? 1 : 0);
+    const u64 rounds = (n + NSTATES - 1) / NSTATES;
     u64 pos = 0;
-    u8 *o = out + l;
-
-    if (K > 64) {
-        // Wide alphabet: serial chain through global tables.  Correct but
-        // slow; no synthetic or fixture stream reaches it.
-        for (u64 r = 0; r < rounds; r++, o += NSTATES) {
-            const u32 cntr = r < full ? NSTATES : rem;
-            u32 symv = 0, ctxv = 0;
-            u32 bad = cntr;
-            for (u32 j = 0; j < cntr; j++) {
-                if (prev == 0xffff) {
-                    bad = j;
-                    break;
-                }
-                const u32 sj = __builtin_amdgcn_readlane(x, j) & (TOTFREQ - 1);
-                const u32 sym = uni(codes[(u64)sj * KS + prev]) & 0xff;
-                if (l == j) {
-                    symv = sym;
-                    ctxv = prev;
-                }
-                prev = sh.dense[sym];
-            }
-            const bool act = l < bad;
-            const u32 fc = act ? fcw[ctxv * 256 + symv] : (1u << 12);
-            const u32 f = fc >> 12, cum = fc & 0xfff;
-            u32 v = f * (x >> SCALE_BITS) + (x & (TOTFREQ - 1)) - cum;
-            if (ballot(act && v == 0)) {
-                int err = slow_round(v, (u32)ballot(act), pos, avail, wr);
-                if (err) return err;
-                wr.reset(sh.ring, pos / RING_CHUNK_WORDS);
-            } else {
-                const bool need = act && v < RANS_L;
-                const u64 m = ballot(need);
-                const u32 k = (u32)__popcll(m);
-                if (pos + k > avail) return FQXV_RANS_ERR_TRUNCATED;
-                const u32 w = wr.word(sh.ring, pos + rank_below(m));
-                if (need) v = (v << 16) | w;
-                pos += k;
-                wr.advance(sh.ring, pos);
-            }
-            if (act) {
-                x = v;
-                *o = (u8)symv;
-            }
-            if (bad < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
-        }
-        return 0;
-    }
-
-    const u16 *mycodes = codes + (l < K ? l : 0);
-    for (u64 r = 0; r < rounds; r++, o += NSTATES) {
-        const u32 cntr = r < full ? NSTATES : rem;
-        const u32 slot = x & (TOTFREQ - 1);
-        // Candidate code of every present context for each of the 32 slots.
-        u32 cand[NSTATES];
-#pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) {
-            const u32 sj = __builtin_amdgcn_readlane(slot, j);
-            cand[j] = mycodes[sj * 64];
-        }
-        // The chain: symbol j's context is symbol j-1.
+    u32 row = 0;
+    u8 *o = out;
+    for (u64 r = 0; r < rounds; r++) {
+        const u32 cntr = (r + 1) * NSTATES <= n ? NSTATES : (u32)(n - r * NSTATES);
         u32 symv = 0, ctxv = 0;
         u32 bad = cntr;
-        u32 cur = prev;
-#pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) {
-            if (j < cntr && bad == cntr) {
-                if (cur >= 64) {
-                    bad = j;
-                } else {
-                    const u32 code = __builtin_amdgcn_readlane(cand[j], cur);
-                    symv = writelane(code & 0xff, j, symv);
-                    ctxv = writelane(cur, j, ctxv);
-                    cur = code >> 8;
-                }
+        for (u32 j = 0; j < cntr; j++) {
+            if (prev == 0xffff) {
+                bad = j;
+                break;
             }
+            const u32 sj = __builtin_amdgcn_readlane(x, j) & (TOTFREQ - 1);
+            const u32 sym = uni(sym16[(u64)sj * 256 + prev]) & 0xff;
+            if (l == j) {
+                symv = sym;
+                ctxv = prev;
+            }
+            prev = sh.dense[sym];
         }
-        prev = cur;
         const bool act = l < bad;
         const u32 fc = act ? fcw[ctxv * 256 + symv] : (1u << 12);
         const u32 f = fc >> 12, cum = fc & 0xfff;
-        u32 v = f * (x >> SCALE_BITS) + slot - cum;
-        if (ballot(act && v == 0)) {
-            int err = slow_round(v, (u32)ballot(act), pos, avail, wr);
-            if (err) return err;
-            wr.reset(sh.ring, pos / RING_CHUNK_WORDS);
-        } else {
-            const bool need = act && v < RANS_L;
-            const u64 m = ballot(need);
-            const u32 k = (u32)__popcll(m);
-            if (pos + k > avail) return FQXV_RANS_ERR_TRUNCATED;
-            const u32 w = wr.word(sh.ring, pos + rank_below(m));
-            if (need) v = (v << 16) | w;
-            pos += k;
-            wr.advance(sh.ring, pos);
-        }
-        if (act) {
-            x = v;
-            *o = (u8)symv;
-        }
+        const u32 v = f * (x >> SCALE_BITS) + (x & (TOTFREQ - 1)) - cum;
+        int err = renorm(sh, wr, x, v, act, pos, avail);
+        if (err) return err;
+        if (act) stage_byte(sh, row, symv);
         if (bad < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        if (++row == OUT_ROUNDS) {
+            // The last round may be partial: never write past out + n.
+            const u64 left = n - (u64)(o - out);
+            flush(sh, o, left < OUT_BYTES ? (u32)left : OUT_BYTES);
+            o += OUT_BYTES;
+            row = 0;
+        }
     }
+    if (row) flush(sh, o, (u32)(n - (u64)(o - out)));
     return 0;
+}
+
+__device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap, u8 *ws)
+{
+    const u32 l = lane_id();
+    u32 K, symp;
+    int err = o1_validate(sh, s, end, K, symp);
+    if (err) return err;
+    const u64 hdr = HDR0 + BITMAP + 512ull * K + 4 * NSTATES;
+    if ((u64)(end - s) < hdr) return FQXV_RANS_ERR_TRUNCATED;
+    const u32 x = read_state(s + hdr - 4 * NSTATES, end);
+    if (n > cap) return FQXV_RANS_ERR_OUTPUT_ALLOC;
+    // Lane space U: contexts first (lane = dense id), then the symbols that
+    // are no context, ascending.
+    u32 extra = 0;
+    for (int i = 0; i < 4; i++) extra += ((symp >> i) & 1) && sh.dense[4 * l + i] == 0xffff;
+    const u32 ex0 = wave_scan_incl(extra) - extra;
+    const u32 nu = K + uni(wave_sum(extra));
+    {
+        u32 e = K + ex0;
+        for (int i = 0; i < 4; i++) {
+            const u32 sI = 4 * l + i;
+            const u32 dd = sh.dense[sI];
+            const bool p = (symp >> i) & 1;
+            sh.ulane[sI] = dd != 0xffff ? (u8)(dd < 255 ? dd : 255) : (p ? (u8)(e < 255 ? e : 255) : (u8)0xff);
+            e += p && dd == 0xffff;
+        }
+    }
+    __syncthreads();
+    if (K <= UMAX && nu <= UMAX) return decode_o1_fast(sh, s, end, out, n, K, x, hdr, ws);
+    return decode_o1_wide(sh, s, end, out, n, K, x, hdr, ws);
 }
 
 // One job: header dispatch (scalar.rs:143-155, lib.rs:226-231).

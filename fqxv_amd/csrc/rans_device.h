@@ -15,6 +15,7 @@ typedef uint16_t u16;
 typedef uint32_t u32;
 typedef uint64_t u64;
 typedef int32_t i32;
+typedef int64_t i64;
 
 // crates/fqxv-rans/src/model.rs:10-19
 constexpr u32 SCALE_BITS = 12;
@@ -112,9 +113,10 @@ __device__ __forceinline__ void wave_argmax(u32 &val, u32 &key)
 // allocation.
 __device__ __forceinline__ u32 load4(const u8 *p, const u8 *end)
 {
-    const uintptr_t a = (uintptr_t)p;
-    const u32 ph = (u32)(a & 3);
-    const u32 *base = (const u32 *)(a & ~(uintptr_t)3);
+    // Pointer arithmetic (not an integer round trip) keeps the global
+    // address space, so these stay global_load rather than flat_load.
+    const u32 ph = (u32)((uintptr_t)p & 3);
+    const u32 *base = (const u32 *)(p - ph);
     if (ph == 0) {
         if (p + 4 <= end) return *base;
     } else if ((const u8 *)(base + 2) <= end) {

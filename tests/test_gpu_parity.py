@@ -186,3 +186,37 @@ def test_workload_mixed_prefix(coder, oracle):
     from fqxv_amd import synth
     c = synth.workload("mixed_10k", blocks=400)
     _corpus_roundtrip(coder, oracle, c, range(0, 400, 7))
+
+
+def test_batched_decode_writes_only_its_region(coder, oracle):
+    """Regression: a block whose last, partial round fills the 32nd row of
+    the output stage must not write past its region.  Regions are packed
+    with 1-byte sentinel gaps at odd offsets; every gap must survive."""
+    import torch
+    rng = np.random.default_rng(12)
+    lens = [993, 1000, 1023, 1024, 1025, 2017, 2048, 31, 33, 5000, 1000, 999]
+    blocks = [rng.integers(0, 30, n).astype(np.uint8).tobytes() for n in lens] * 4
+    orders = [(i // len(lens)) & 1 for i in range(len(blocks))]
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    offs = np.zeros(len(blocks), dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(blocks) + bytes(64), dtype=np.uint8).copy()).cuda()
+    r = coder.encode_batch(arena, offs, sizes, orders)
+    torch.cuda.synchronize()
+    assert np.all(r.status.cpu().numpy() == 0)
+    ol = r.out_len.cpu().numpy()
+    doff = np.zeros(len(blocks), dtype=np.uint64)
+    acc = 3
+    for j, n in enumerate(sizes):
+        doff[j] = acc
+        acc += int(n) + 1
+    dst = torch.full((acc + 64,), 0xEE, dtype=torch.uint8, device="cuda")
+    d = coder.decode_batch(r.dst, r.dst_off, ol, sizes, dst=dst, dst_off=doff)
+    torch.cuda.synchronize()
+    assert np.all(d.status.cpu().numpy() == 0)
+    out = dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        o = int(doff[j])
+        assert out[o:o + len(b)].tobytes() == b, j
+        assert out[o + len(b)] == 0xEE, f"block {j} wrote past its region"
+    assert np.all(out[:3] == 0xEE)

@@ -37,6 +37,7 @@ constexpr u32 RING_MIRROR = 32;                      // words mirrored past the 
 constexpr u32 OUT_ROUNDS = 32;                       // rounds per output flush
 constexpr u32 OUT_BYTES = OUT_ROUNDS * NSTATES;      // 1 KiB
 constexpr u32 UMAX = 64;                             // order-1 fast path lane space
+constexpr u32 FAST_ROUNDS = 4;                       // rounds per branch-free block
 
 struct DecShared {
     u32 tab[TOTFREQ];                                 // o0 slot table; o1: fc2[64][64]
@@ -267,11 +268,53 @@ __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     u64 pos = 0;
     u32 row = 0;
     u8 *o = out;
-    for (u64 r = 0; r < full; r++) {
+    u64 r = 0;
+    while (r < full) {
+        if (r + FAST_ROUNDS <= full && pos + FAST_ROUNDS * NSTATES <= avail && (row % FAST_ROUNDS) == 0) {
+            // Fast block: FAST_ROUNDS rounds with no branch inside.  No
+            // truncation is possible (enough words remain), the ring holds
+            // every word the block can reach (at most 128 words, the ring
+            // keeps 512 ahead of the chunk start), and a state that would
+            // need a second word (only v == 0 does, and only then is the
+            // renormalised state below 2^16) sends the whole block back to
+            // the exact per-round path below.
+            const u32 x0 = x;
+            const u64 pos0 = pos;
+            u64 bad = 0;
+#pragma unroll
+            for (u32 t = 0; t < FAST_ROUNDS; t++) {
+                const u32 p = sh.tab[x & (TOTFREQ - 1)];
+                const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
+                const bool need = lo && v < RANS_L;
+                const u64 m = ballot(need);
+                const u32 w = ((const u16 *)sh.ring)[(u32)(pos % RING_WORDS) +
+                                                     __builtin_amdgcn_mbcnt_lo((u32)m, 0u)];
+                if (lo) x = need ? ((v << 16) | w) : v;
+                pos += (u32)__popcll(m);
+                if (lo) stage_byte(sh, row + t, p);
+                bad |= ballot(lo && x < RANS_L);
+            }
+            if (bad) {
+                x = x0;
+                pos = pos0;
+            } else {
+                r += FAST_ROUNDS;
+                row += FAST_ROUNDS;
+                wr.advance(sh.ring, pos);
+                if (row == OUT_ROUNDS) {
+                    flush(sh, o, OUT_BYTES);
+                    o += OUT_BYTES;
+                    row = 0;
+                }
+                continue;
+            }
+        }
+        // Exact per-round path: truncation, multi-word states, ring refill.
         const u32 p = sh.tab[x & (TOTFREQ - 1)];
         const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
         if ((err = renorm(sh, wr, x, v, lo, pos, avail))) return err;
         if (lo) stage_byte(sh, row, p);
+        r++;
         if (++row == OUT_ROUNDS) {
             flush(sh, o, OUT_BYTES);
             o += OUT_BYTES;
@@ -322,13 +365,19 @@ __device__ int o1_validate(DecShared &sh, const u8 *s, const u8 *end, u32 &K, u3
 }
 
 // Fast path: K <= 64 contexts and |contexts u symbols| <= 64.
-// Workspace: t8[slot * kp + d] (u8) = lane(sym) | 0x40 when sym is no context.
-// LDS: fc2[d * 64 + lane(sym)] = (f-1) << 20 | cum << 8 | sym.
+// Workspace: t8[slot * K + d] (u8) = lane(sym), followed by 64 zero bytes, so
+// every lane of a gather (also lanes >= K, which read the next row) gets a
+// valid lane index.  LDS: fc2[d * 64 + lane(sym)] = (f-1) << 20 | cum << 8 | sym.
+//
+// Per round lane c gathers, for each of the 32 slots, the lane of the symbol
+// context c decodes there (cand[j]); the chain then walks 32 dependent
+// v_readlane steps from the round's first context.  (Composing the maps
+// pairwise with ds_bpermute cuts the serial depth but costs more issue slots
+// than it saves: measured slower on gfx950.)
 __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
                               u64 hdr, u8 *ws)
 {
     const u32 l = lane_id();
-    const u32 kp = (K + 3) & ~3u;
     u8 *t8 = ws;
     u32 *fc2 = sh.tab;
     const u8 *t = s + HDR0 + BITMAP;
@@ -336,11 +385,12 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
         (void)load_freq_table(sh, t, end); // validated already
         for_each_slot(sh, [&](u32 k, u32 sym, u32 f, u32 cum) {
             const u32 ul = sh.ulane[sym];
-            t8[(u64)k * kp + d] = (u8)(ul | (ul >= K ? 0x40u : 0u));
+            t8[k * K + d] = (u8)ul;
             if (k == cum) fc2[d * UMAX + ul] = ((f - 1) << 20) | (cum << 8) | sym;
         });
         __syncthreads();
     }
+    t8[TOTFREQ * K + l] = 0;
     __threadfence();
 
     WordRing wr;
@@ -348,39 +398,39 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     const u64 avail = ((u64)(end - s) - hdr) / 2;
     wr.reset(sh.ring, 0);
 
-    u32 prev = sh.dense[0];
-    if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT; // context 0 of symbol 0 (scalar.rs:203)
+    u32 prev = sh.dense[0]; // context of symbol 0 is 0 (scalar.rs:203)
     const bool lo = l < NSTATES;
     const u64 rounds = (n + NSTATES - 1) / NSTATES;
     u64 pos = 0;
     u32 row = 0;
     u8 *o = out;
-    const u8 *mycol = t8 + l;
     for (u64 r = 0; r < rounds; r++) {
+        if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT;
         const u32 cntr = (r + 1) * NSTATES <= n ? NSTATES : (u32)(n - r * NSTATES);
         const u32 slot = x & (TOTFREQ - 1);
         if (lo) sh.sbuf[l] = slot;
         u32 cand[NSTATES];
 #pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) cand[j] = mycol[(u64)sh.sbuf[j] * kp];
-        u32 u = prev, cv = 0;
+        for (u32 j = 0; j < NSTATES; j++) cand[j] = t8[sh.sbuf[j] * K + l];
+        // The chain: 32 dependent v_readlane steps (about 25 cycles each).
+        u32 c[NSTATES + 1];
+        c[0] = prev;
 #pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) {
-            const u32 e = __builtin_amdgcn_readlane(cand[j], u);
-            cv = writelane(u | (e << 8), j, cv);
-            u = e & 0x3f;
-        }
-        prev = u;
-        // First position whose symbol is no context: the next symbol fails.
-        const u64 nb = ballot(lo && l < cntr && ((cv >> 8) & 0x40));
+        for (u32 j = 0; j < NSTATES; j++) c[j + 1] = __builtin_amdgcn_readlane(cand[j], c[j]);
+        u32 cv = 0;
+#pragma unroll
+        for (u32 j = 0; j < NSTATES; j++) cv = writelane(c[j] | (c[j + 1] << 8), j, cv);
+        prev = c[NSTATES];
+        // First position whose context is no context: that symbol fails.
+        const u64 nb = ballot(lo && l < cntr && (cv & 0xff) >= K);
         const u32 jb = nb ? (u32)__builtin_ctzll(nb) : NSTATES;
-        const bool act = lo && l < cntr && l <= jb;
-        const u32 ent = fc2[(cv & 0xff) * UMAX + ((cv >> 8) & 0x3f)];
+        const bool act = lo && l < cntr && l < jb;
+        const u32 ent = fc2[(cv & 0x3f) * UMAX + ((cv >> 8) & 0x3f)];
         const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
         int err = renorm(sh, wr, x, v, act, pos, avail);
         if (err) return err;
         if (act) stage_byte(sh, row, ent);
-        if (jb < NSTATES && (u64)r * NSTATES + jb + 1 < n) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
         if (++row == OUT_ROUNDS) {
             // The last round may be partial: never write past out + n.
             const u64 left = n - (u64)(o - out);

@@ -12,10 +12,10 @@
 //   * renormalisation words are consumed in lane order: a ballot marks the
 //     lanes with x' < 2^16, v_mbcnt gives each its rank, and lane l takes
 //     word[pos + rank] from an LDS ring refilled one 512-byte chunk at a
-//     time.  The refill loads raw aligned dwords one chunk ahead (no waits,
-//     no branches) and realigns them with v_alignbyte when they are written
-//     to LDS at the next chunk boundary; the ring's first 32 words are
-//     mirrored past its end so a round's reads never wrap;
+//     time, two chunks ahead of the read position.  The refill loads raw
+//     aligned dwords (no branches) and realigns them with v_alignbyte; the
+//     ring's first 32 words are mirrored past its end so a round's reads
+//     never wrap;
 //   * decoded bytes are staged in LDS and written 1 KiB (32 rounds) at a
 //     time with one 16-byte store per lane;
 //   * order-1: symbol i's context is symbol i-1 (scalar.rs:203), a serial
@@ -52,8 +52,10 @@ struct DecShared {
 };
 
 // Words region reader (see the file comment).  Chunk c covers words
-// [256c, 256c + 256).  When the read position enters chunk c, chunks c, c+1,
-// c+2 are resident and chunk c+3's raw dwords are in flight in r0..r3.
+// [256c, 256c + 256).  While the read position is in chunk c, chunks c, c+1
+// and c+2 are resident; entering chunk c+1 loads chunk c+3.  The refill is
+// synchronous: keeping its loads in flight across rounds measured slower,
+// because later waits on unrelated loads then also wait for them.
 struct WordRing {
     const u32 *A;  // 4-byte-aligned address at or below the words region
     u32 ph;        // byte offset of the region within A[0]
@@ -95,15 +97,14 @@ struct WordRing {
             fetch(c);
             put(ring, c);
         }
-        fetch(c0 + 3);
         __syncthreads();
     }
     __device__ __forceinline__ void advance(u32 *ring, u64 pos)
     {
         if ((pos / RING_CHUNK_WORDS) != chunk) {
             chunk++;
+            fetch(chunk + 2);
             put(ring, chunk + 2);
-            fetch(chunk + 3);
         }
     }
     // Word pos + rank, rank < 32.
@@ -370,10 +371,10 @@ __device__ int o1_validate(DecShared &sh, const u8 *s, const u8 *end, u32 &K, u3
 // valid lane index.  LDS: fc2[d * 64 + lane(sym)] = (f-1) << 20 | cum << 8 | sym.
 //
 // Per round lane c gathers, for each of the 32 slots, the lane of the symbol
-// context c decodes there (cand[j]); the chain then walks 32 dependent
-// v_readlane steps from the round's first context.  (Composing the maps
-// pairwise with ds_bpermute cuts the serial depth but costs more issue slots
-// than it saves: measured slower on gfx950.)
+// context c decodes there; the chain then walks 32 dependent v_readlane
+// steps from the round's first context.  (Composing
+// the maps pairwise with ds_bpermute cuts the serial depth but costs more
+// issue slots than it saves: measured slower on gfx950.)
 __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
                               u64 hdr, u8 *ws)
 {
@@ -401,26 +402,70 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     u32 prev = sh.dense[0]; // context of symbol 0 is 0 (scalar.rs:203)
     const bool lo = l < NSTATES;
     const u64 rounds = (n + NSTATES - 1) / NSTATES;
+    const u64 full = n / NSTATES;
     u64 pos = 0;
     u32 row = 0;
     u8 *o = out;
-    for (u64 r = 0; r < rounds; r++) {
-        if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT;
-        const u32 cntr = (r + 1) * NSTATES <= n ? NSTATES : (u32)(n - r * NSTATES);
+    u64 r = 0;
+    while (r < rounds) {
         const u32 slot = x & (TOTFREQ - 1);
-        if (lo) sh.sbuf[l] = slot;
+        if (lo) sh.sbuf[l] = slot * K;
         u32 cand[NSTATES];
+        // Row offsets are broadcast through LDS; lane l reads column l.
 #pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) cand[j] = t8[sh.sbuf[j] * K + l];
-        // The chain: 32 dependent v_readlane steps (about 25 cycles each).
-        u32 c[NSTATES + 1];
-        c[0] = prev;
+        for (u32 j = 0; j < NSTATES; j++) cand[j] = t8[sh.sbuf[j] + l];
+        if (r < full && prev < K && pos + NSTATES <= avail) {
+            // Fast round, no branch inside: a full round, a valid first
+            // context, no truncation possible and the ring holds every word
+            // the round can reach.  An absent context or a state needing a
+            // second word sends the round back to the exact path below.
+            const u32 x0 = x;
+            const u64 pos0 = pos;
+            u32 c = prev;
+            u32 sv = 0;
 #pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) c[j + 1] = __builtin_amdgcn_readlane(cand[j], c[j]);
+            for (u32 j = 0; j < NSTATES; j++) {
+                c = __builtin_amdgcn_readlane(cand[j], c);
+                sv = writelane(c, j, sv);
+            }
+            // Position j's context is position j-1's symbol (prev for j = 0).
+            u32 ctxv = __shfl_up(sv, 1);
+            if (l == 0) ctxv = prev;
+            const u32 cv = ctxv | (sv << 8);
+            const u32 ent = fc2[ctxv * UMAX + sv];
+            const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
+            const bool need = lo && v < RANS_L;
+            const u64 m = ballot(need);
+            const u32 w = ((const u16 *)sh.ring)[(u32)(pos % RING_WORDS) + __builtin_amdgcn_mbcnt_lo((u32)m, 0u)];
+            if (lo) x = need ? ((v << 16) | w) : v;
+            pos += (u32)__popcll(m);
+            if (lo) stage_byte(sh, row, ent);
+            if (ballot(lo && (x < RANS_L || (cv & 0xff) >= K)) == 0) {
+                prev = c;
+                r++;
+                wr.advance(sh.ring, pos);
+                if (++row == OUT_ROUNDS) {
+                    flush(sh, o, OUT_BYTES);
+                    o += OUT_BYTES;
+                    row = 0;
+                }
+                continue;
+            }
+            x = x0;
+            pos = pos0;
+        }
+        // Exact path: absent contexts, truncation, multi-word states, tail.
+        if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        const u32 cntr = r < full ? NSTATES : (u32)(n - r * NSTATES);
+        u32 c = prev;
         u32 cv = 0;
 #pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) cv = writelane(c[j] | (c[j + 1] << 8), j, cv);
-        prev = c[NSTATES];
+        for (u32 j = 0; j < NSTATES; j++) {
+            const u32 nxt = __builtin_amdgcn_readlane(cand[j], c);
+            cv = writelane(c | (nxt << 8), j, cv);
+            c = nxt;
+        }
+        prev = c;
         // First position whose context is no context: that symbol fails.
         const u64 nb = ballot(lo && l < cntr && (cv & 0xff) >= K);
         const u32 jb = nb ? (u32)__builtin_ctzll(nb) : NSTATES;
@@ -431,6 +476,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
         if (err) return err;
         if (act) stage_byte(sh, row, ent);
         if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        r++;
         if (++row == OUT_ROUNDS) {
             // The last round may be partial: never write past out + n.
             const u64 left = n - (u64)(o - out);

@@ -5,13 +5,13 @@ from ``fqxv_amd/csrc`` (hand-written HIP kernels for gfx950); this package is
 its Python front-end mirroring crates/fqxv-rans/src/lib.rs.
 """
 from .rans import (  # noqa: F401
-    ALLOC_LIMIT, Backend, BatchResult, Coder, DeviceError, Error, InputTooLarge,
+    ALLOC_LIMIT, Backend, BatchResult, Coder, DeviceError, Error, Graph, InputTooLarge,
     Malformed, Order, decode, decode_bounded, decode_with, default_coder, encode,
     peek_output_len,
 )
 
 __all__ = [
-    "Backend", "BatchResult", "Coder", "DeviceError", "Error", "InputTooLarge",
+    "Backend", "BatchResult", "Coder", "DeviceError", "Error", "Graph", "InputTooLarge",
     "Malformed", "Order", "decode", "decode_bounded", "decode_with", "encode",
     "peek_output_len", "default_coder", "ALLOC_LIMIT",
 ]

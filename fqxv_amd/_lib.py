@@ -32,6 +32,8 @@ EXPORTS = [
     "fqxv_rans_decode", "fqxv_rans_decode_bounded", "fqxv_rans_decode_batch",
     "fqxv_rans_encode_batch", "fqxv_rans_decode_one_device",
     "fqxv_rans_encode_one_device", "fqxv_rans_lpt_order", "fqxv_rans_max_waves",
+    "fqxv_rans_graph_create", "fqxv_rans_graph_launch", "fqxv_rans_graph_destroy",
+    "fqxv_rans_ctx_set_grid",
 ]
 
 
@@ -80,6 +82,12 @@ def load():
         L.fqxv_rans_lpt_order.argtypes = [vp, ctypes.c_uint32, vp]
         L.fqxv_rans_lpt_order.restype = None
         L.fqxv_rans_max_waves.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.fqxv_rans_graph_create.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_uint32, vp, vp, vp,
+                                             ctypes.POINTER(vp)]
+        L.fqxv_rans_graph_launch.argtypes = [vp, vp]
+        L.fqxv_rans_graph_destroy.argtypes = [vp]
+        L.fqxv_rans_graph_destroy.restype = None
+        L.fqxv_rans_ctx_set_grid.argtypes = [vp, ctypes.c_uint32]
         _lib = L
         return L
 

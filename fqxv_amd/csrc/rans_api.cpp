@@ -49,7 +49,8 @@ struct fqxv_rans_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    u32 dec_waves = 0, enc_waves = 0;
+    u32 dec_waves = 0, enc_waves = 0;       // grid used by batched calls
+    u32 dec_max = 0, enc_max = 0;           // occupancy maxima
     u8 *ws = nullptr;
     u64 ws_waves = 0;
     u32 *counters = nullptr;
@@ -182,8 +183,8 @@ int fqxv_rans_ctx_create(int device, fqxv_rans_ctx **out)
         fqxv_rans_ctx_destroy(c);
         return FQXV_RANS_ERR_DEVICE;
     }
-    c->dec_waves = (u32)std::max(1, dec_occ) * (u32)prop.multiProcessorCount;
-    c->enc_waves = (u32)std::max(1, enc_occ) * (u32)prop.multiProcessorCount;
+    c->dec_waves = c->dec_max = (u32)std::max(1, dec_occ) * (u32)prop.multiProcessorCount;
+    c->enc_waves = c->enc_max = (u32)std::max(1, enc_occ) * (u32)prop.multiProcessorCount;
     // FQXV_RANS_GRID caps the persistent grid (tuning and tests).
     if (const char *g = getenv("FQXV_RANS_GRID")) {
         const long v = strtol(g, nullptr, 10);
@@ -217,7 +218,16 @@ void fqxv_rans_ctx_destroy(fqxv_rans_ctx *c)
 int fqxv_rans_max_waves(fqxv_rans_ctx *c, uint32_t *waves)
 {
     if (!c || !waves) return FQXV_RANS_ERR_INVALID_ARG;
-    *waves = std::max(c->dec_waves, c->enc_waves);
+    *waves = std::max(c->dec_max, c->enc_max);
+    return FQXV_RANS_OK;
+}
+
+int fqxv_rans_ctx_set_grid(fqxv_rans_ctx *c, uint32_t waves)
+{
+    if (!c) return FQXV_RANS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->dec_waves = waves ? std::min(waves, c->dec_max) : c->dec_max;
+    c->enc_waves = waves ? std::min(waves, c->enc_max) : c->enc_max;
     return FQXV_RANS_OK;
 }
 
@@ -343,6 +353,73 @@ static int one_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, co
     const hipError_t e = enc ? fqxv_launch_encode_one(d_src, d_dst, *job, d_status, d_out_len, c->ws, st)
                              : fqxv_launch_decode_one(d_src, d_dst, *job, d_status, d_out_len, c->ws, st);
     return e == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
+}
+
+struct fqxv_rans_graph {
+    fqxv_rans_ctx *ctx = nullptr;
+    u32 *counter = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+void fqxv_rans_graph_destroy(fqxv_rans_graph *g)
+{
+    if (!g) return;
+    if (g->ctx) {
+        DeviceGuard dg(g->ctx->device);
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        if (g->counter) (void)hipFree(g->counter);
+    }
+    delete g;
+}
+
+int fqxv_rans_graph_create(fqxv_rans_ctx *c, int direction, const uint8_t *d_src, uint8_t *d_dst,
+                           const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
+                           int32_t *d_status, uint64_t *d_out_len, fqxv_rans_graph **out)
+{
+    if (!c || !out || (direction != 0 && direction != 1) || count == 0 || !d_src || !d_dst || !d_jobs ||
+        !d_status || !d_out_len)
+        return FQXV_RANS_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    const bool enc = direction == 1;
+    const u32 grid = std::min<u32>(count, enc ? c->enc_waves : c->dec_waves);
+    // The graph bakes in the workspace address: size it for the largest grid
+    // any later call on this context can ask for, so it is never reallocated.
+    int rc = ensure_ws(c, std::max(c->dec_max, c->enc_max));
+    if (rc) return rc;
+    auto *g = new (std::nothrow) fqxv_rans_graph();
+    if (!g) return FQXV_RANS_ERR_DEVICE;
+    g->ctx = c;
+    hipStream_t cs = nullptr;
+    bool ok = hipMalloc((void **)&g->counter, sizeof(u32)) == hipSuccess && hipStreamCreate(&cs) == hipSuccess &&
+              hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        const hipError_t e = enc ? fqxv_launch_encode_batch(d_src, d_dst, d_jobs, count, d_order, g->counter,
+                                                            d_status, d_out_len, c->ws, grid, cs)
+                                 : fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, g->counter,
+                                                            d_status, d_out_len, c->ws, grid, cs);
+        const hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+        ok = e == hipSuccess && e2 == hipSuccess &&
+             hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) == hipSuccess;
+    }
+    if (cs) (void)hipStreamDestroy(cs);
+    if (!ok) {
+        fqxv_rans_graph_destroy(g);
+        return FQXV_RANS_ERR_DEVICE;
+    }
+    *out = g;
+    return FQXV_RANS_OK;
+}
+
+int fqxv_rans_graph_launch(fqxv_rans_graph *g, void *stream)
+{
+    if (!g || !g->exec) return FQXV_RANS_ERR_INVALID_ARG;
+    DeviceGuard dg(g->ctx->device);
+    return hipGraphLaunch(g->exec, pick_stream(g->ctx, stream)) == hipSuccess ? FQXV_RANS_OK
+                                                                             : FQXV_RANS_ERR_DEVICE;
 }
 
 int fqxv_rans_decode_one_device(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,

@@ -269,6 +269,58 @@ class Coder:
         _raise(self._L.fqxv_rans_max_waves(self._h, ctypes.byref(w)))
         return int(w.value)
 
+    def set_grid(self, waves: int) -> None:
+        """Persistent-grid size of later batched calls (0 = occupancy max)."""
+        _raise(self._L.fqxv_rans_ctx_set_grid(self._h, int(waves)))
+
+    def graph(self, direction: str, src, dst, jobs_np: np.ndarray, order_np=None) -> "Graph":
+        """Capture one batched call (``"decode"`` or ``"encode"``) over fixed
+        device buffers into a hipGraph; :meth:`Graph.launch` replays it."""
+        import torch
+        d = {"decode": 0, "encode": 1}[direction]
+        dev = src.device
+        jobs = torch.from_numpy(jobs_np.view(np.uint8).copy()).to(dev)
+        k = len(jobs_np)
+        status = torch.full((k,), -1, dtype=torch.int32, device=dev)
+        out_len = torch.zeros((k,), dtype=torch.int64, device=dev)
+        ordt = None
+        if order_np is not None:
+            ordt = torch.from_numpy(np.ascontiguousarray(order_np, dtype=np.uint32).view(np.int32)).to(dev)
+        torch.cuda.synchronize(dev)
+        h = ctypes.c_void_p()
+        rc = self._L.fqxv_rans_graph_create(
+            self._h, d, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+            ctypes.c_void_p(jobs.data_ptr()), k, ctypes.c_void_p(ordt.data_ptr()) if ordt is not None else None,
+            ctypes.c_void_p(status.data_ptr()), ctypes.c_void_p(out_len.data_ptr()), ctypes.byref(h))
+        _raise(rc)
+        return Graph(self, h, src, dst, jobs, ordt, status, out_len)
+
+
+class Graph:
+    """A captured batched call; buffers stay referenced while it lives."""
+
+    def __init__(self, coder: Coder, h, src, dst, jobs, ordt, status, out_len):
+        self._coder = coder
+        self._h = h
+        self._keep = (src, jobs, ordt)
+        self.dst = dst
+        self.status = status
+        self.out_len = out_len
+
+    def launch(self, stream=None) -> None:
+        _raise(self._coder._L.fqxv_rans_graph_launch(self._h, Coder._stream(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._coder._L.fqxv_rans_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 _tls = threading.local()
 

@@ -131,6 +131,26 @@ int fqxv_rans_encode_one_device(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_
                                 const fqxv_rans_job *job, int32_t *d_status, uint64_t *d_out_len,
                                 void *stream);
 
+/* Batched call captured once into a hipGraph and replayed: the work-queue
+ * reset kernel and the persistent kernel, arguments fixed at creation (the
+ * graph owns its own queue counter).  `direction` is 0 for decode, 1 for
+ * encode.  No reference counterpart: the form the container's per-block
+ * dispatch (decompress.rs:45-52) takes when the same layout is decoded
+ * repeatedly.  A replay must not overlap another call on the same context;
+ * the context must outlive its graphs.  The buffers passed at creation are
+ * read and written by every replay. */
+typedef struct fqxv_rans_graph fqxv_rans_graph;
+int fqxv_rans_graph_create(fqxv_rans_ctx *ctx, int direction, const uint8_t *d_src, uint8_t *d_dst,
+                           const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
+                           int32_t *d_status, uint64_t *d_out_len, fqxv_rans_graph **out);
+int fqxv_rans_graph_launch(fqxv_rans_graph *g, void *stream);
+void fqxv_rans_graph_destroy(fqxv_rans_graph *g);
+
+/* Persistent-grid size for later batched calls and graphs on `ctx`
+ * (workgroups of one wave; 0 restores the occupancy maximum).  Used by the
+ * autotuner. */
+int fqxv_rans_ctx_set_grid(fqxv_rans_ctx *ctx, uint32_t waves);
+
 /* Host helper: job order, largest `sizes[i]` first (ties by index). */
 void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);
 

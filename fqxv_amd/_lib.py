@@ -33,7 +33,9 @@ EXPORTS = [
     "fqxv_rans_encode_batch", "fqxv_rans_decode_one_device",
     "fqxv_rans_encode_one_device", "fqxv_rans_lpt_order", "fqxv_rans_max_waves",
     "fqxv_rans_graph_create", "fqxv_rans_graph_launch", "fqxv_rans_graph_destroy",
-    "fqxv_rans_ctx_set_grid",
+    "fqxv_rans_ctx_set_grid", "fqxv_rans_server_create", "fqxv_rans_server_destroy",
+    "fqxv_rans_server_now", "fqxv_rans_server_submit",
+    "fqxv_rans_server_poll", "fqxv_rans_server_take", "fqxv_rans_server_release",
 ]
 
 
@@ -43,6 +45,14 @@ class Job(ctypes.Structure):
                 ("dst_off", ctypes.c_uint64), ("dst_cap", ctypes.c_uint64),
                 ("max_len", ctypes.c_uint64), ("order", ctypes.c_int32),
                 ("flags", ctypes.c_uint32)]
+
+
+class Done(ctypes.Structure):
+    """fqxv_rans_done (56 bytes)."""
+    _fields_ = [("ticket", ctypes.c_uint64), ("out_off", ctypes.c_uint64),
+                ("out_bytes", ctypes.c_uint64), ("t_arrival", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("status", ctypes.c_int32), ("nblocks", ctypes.c_uint32)]
 
 
 JOB_DTYPE_FIELDS = [("src_off", "<u8"), ("src_len", "<u8"), ("dst_off", "<u8"),
@@ -88,6 +98,16 @@ def load():
         L.fqxv_rans_graph_destroy.argtypes = [vp]
         L.fqxv_rans_graph_destroy.restype = None
         L.fqxv_rans_ctx_set_grid.argtypes = [vp, ctypes.c_uint32]
+        u32 = ctypes.c_uint32
+        L.fqxv_rans_server_create.argtypes = [ctypes.c_int, vp, vp, u32, vp, ctypes.c_uint64, u32, u32,
+                                              ctypes.POINTER(vp)]
+        L.fqxv_rans_server_destroy.argtypes = [vp]
+        L.fqxv_rans_server_destroy.restype = None
+        L.fqxv_rans_server_now.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.fqxv_rans_server_submit.argtypes = [vp, vp, u32, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]
+        L.fqxv_rans_server_poll.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+        L.fqxv_rans_server_take.argtypes = [vp, ctypes.POINTER(Done), u32, ctypes.POINTER(u32)]
+        L.fqxv_rans_server_release.argtypes = [vp, ctypes.c_uint64]
         _lib = L
         return L
 

@@ -151,6 +151,48 @@ void fqxv_rans_graph_destroy(fqxv_rans_graph *g);
  * autotuner. */
 int fqxv_rans_ctx_set_grid(fqxv_rans_ctx *ctx, uint32_t waves);
 
+/* ---- Serving: random-access decode from an archive resident in HBM ------
+ *
+ * No reference counterpart (the reference decodes whole containers,
+ * decompress.rs:45-52); this is the request loop of a decode service.
+ * `blocks[i]` (host array, copied) locates block i of the resident archive
+ * `d_archive`: src_off/src_len of its stream, dst_cap = its decoded length n,
+ * max_len the decode_bounded bound.  Decoded output lives in the caller's
+ * device buffer `d_pool` of `budget` bytes; a request is admitted (in arrival
+ * order) when its output fits in the free pool and a slot is free.  Every
+ * admitted request that fits is packed into one batched decode of at most
+ * `max_batch` blocks; up to `slots` such launches run at once, each on its own
+ * stream.  Output of a request: its blocks in request order, block i at
+ * d_pool + out_off + sum over earlier blocks of round_up(n, 256).  The
+ * region stays valid until
+ * fqxv_rans_server_release(ticket).  Times are seconds since creation. */
+typedef struct fqxv_rans_server fqxv_rans_server;
+typedef struct fqxv_rans_done {
+    uint64_t ticket;
+    uint64_t out_off;   /* offset of the request's output in the pool */
+    uint64_t out_bytes; /* bytes of pool held by the request */
+    double t_arrival;   /* as passed to submit (or the submit time) */
+    double t_admit;     /* launch of the batch that carried it */
+    double t_done;      /* completion seen by fqxv_rans_server_poll */
+    int32_t status;     /* FQXV_RANS_OK or the first failing block's code */
+    uint32_t nblocks;
+} fqxv_rans_done;
+
+int fqxv_rans_server_create(int device, const uint8_t *d_archive, const fqxv_rans_job *blocks, uint32_t nblocks,
+                            uint8_t *d_pool, uint64_t budget, uint32_t slots, uint32_t max_batch,
+                            fqxv_rans_server **out);
+void fqxv_rans_server_destroy(fqxv_rans_server *srv);
+int fqxv_rans_server_now(fqxv_rans_server *srv, double *t);
+/* Queue a request for blocks ids[0..k-1]; t_arrival < 0 means now. */
+int fqxv_rans_server_submit(fqxv_rans_server *srv, const uint32_t *ids, uint32_t k, double t_arrival,
+                            uint64_t *ticket);
+/* Progress: completes finished launches, then admits queued requests. */
+int fqxv_rans_server_poll(fqxv_rans_server *srv, uint32_t *queued, uint32_t *in_flight);
+/* Pops up to `max` completion records. */
+int fqxv_rans_server_take(fqxv_rans_server *srv, fqxv_rans_done *out, uint32_t max, uint32_t *got);
+/* Returns a completed request's output region to the pool. */
+int fqxv_rans_server_release(fqxv_rans_server *srv, uint64_t ticket);
+
 /* Host helper: job order, largest `sizes[i]` first (ties by index). */
 void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);
 

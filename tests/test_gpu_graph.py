@@ -124,3 +124,36 @@ def test_grid_size_does_not_change_output(coder, oracle, waves):
         assert torch.equal(d.dst[:n], arena[:n])
     finally:
         coder.set_grid(0)
+
+
+def test_autotune_picks_a_measured_candidate_and_caches(coder, tmp_path, monkeypatch):
+    import torch
+    from fqxv_amd import tune
+    monkeypatch.setenv("FQXV_RANS_TUNE_CACHE", str(tmp_path / "tune.json"))
+    blocks = _blocks()
+    orders = np.array([i & 1 for i in range(len(blocks))], dtype=np.int32)
+    arena, offs, sizes = _arena(blocks)
+    enc = coder.encode_batch(arena, offs, sizes, orders)
+    torch.cuda.synchronize()
+    ol = enc.out_len.cpu().numpy().astype(np.uint64)
+    dst_off = np.zeros_like(sizes)
+    dst_off[1:] = np.cumsum(sizes)[:-1]
+    dst = torch.empty(int(sizes.sum()) + 64, dtype=torch.uint8, device="cuda")
+    jobs = coder.jobs_array(enc.dst_off, ol, dst_off, sizes)
+    try:
+        t = tune.autotune(coder, "decode", enc.dst, dst, jobs, sizes, orders, grids=[40, 8], reps=1)
+        assert (t.grid, t.policy, t.ms) in [tuple(r) for r in t.table]
+        assert len(t.table) == 2 * len(tune.POLICIES)
+        assert (tmp_path / "tune.json").exists()
+        again = tune.autotune(coder, "decode", enc.dst, dst, jobs, sizes, orders, grids=[40, 8], reps=1)
+        assert (again.grid, again.policy) == (t.grid, t.policy)
+        g = coder.graph("decode", enc.dst, dst, jobs, t.order(coder, sizes, orders))
+        dst.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        n = int(sizes.sum())
+        assert np.all(g.status.cpu().numpy() == 0)
+        assert torch.equal(dst[:n], arena[:n])
+        g.close()
+    finally:
+        coder.set_grid(0)

@@ -1,26 +1,39 @@
 #!/usr/bin/env python3
 """rANS Nx16 decode throughput on one MI355X: batched vs per-block launch.
 
-Workload (default ``mixed_10k``): 10,000 blocks, sizes log-uniform on
-[4 KiB, 4 MiB] (6.02 GB), order-0/1 at 50/50 per block, streams split in
-thirds between Markov quality strings, DNA (ACGT + 0.1% N) and ':'-separated
-decimal name fields; seed 4.  The blocks are generated on the host, copied to
-HBM, encoded on the device (byte identity against the oracle is checked on a
-sample of blocks, and the device round trip on all of them), then:
+Headline workload (``value``): ``mixed_10k`` -- 10,000 blocks, sizes
+log-uniform on [4 KiB, 4 MiB] (6.02 GB), order-0/1 at 50/50 per block,
+streams split in thirds between Markov quality strings, DNA (ACGT + 0.1% N)
+and ':'-separated decimal name fields; seed 4.  The blocks are generated on
+the host, copied to HBM and encoded on the device (byte identity against the
+oracle on a sample of blocks, device round trip on all of them).  One step =
+one replay of a captured hipGraph holding the batched decode of the whole
+corpus (one persistent-kernel launch over all blocks from a device work
+queue, grid and queue order chosen by the autotuner), inputs resident in HBM;
+GB/s = 1e9 bytes of uncompressed output per second.
 
-* ``value``: one step = one batched decode of the whole corpus (one
-  persistent-kernel launch over all blocks, inputs resident in HBM);
-  GB/s = 1e9 bytes of uncompressed output per second.
-* ``perblock``: the plain form -- one kernel launch per block -- timed on an
-  every-k-th-block sample, with per-launch p50/p99 latency.
+Also reported on the same JSON line:
+
+* ``perblock``: the plain form -- one kernel launch per block, back to back
+  on one stream -- on an every-k-th-block sample, with per-launch p50/p99.
+* ``configs``: 1 MiB binned order-0 (encode+decode round trip); 64 x 1 MiB
+  normal-quality order-0 and 64 x 4 MiB Markov order-1, each in the plain
+  per-block form (encode and decode) and batched, with the per-round
+  critical-path time of one block.
+* ``serving``: the Markov order-1 corpus resident compressed in HBM, Poisson
+  arrivals of 1-64 block requests (seed 5) swept from 10% to 120% of the
+  batched peak under a 32 GiB output budget: p50/p99 latency and GB/s.
 * ``roofline``: HBM-bound; algorithmic bytes = compressed bytes read +
-  uncompressed bytes written, over the decode kernel's measured duration.
+  uncompressed bytes written, over the decode kernel's measured duration;
+  ``traffic`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs.
 * ``cpu_baseline``: the oracle (plain-C scalar restatement of fqxv-rans)
-  decoding a sample of the same streams on the host's cores.
+  decoding a sample of the same streams on the host's cores, all cores
+  (block-parallel, like the reference's rayon dispatch) and one thread.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -33,10 +46,373 @@ sys.path.insert(0, ROOT)
 
 METRIC = "rANS Nx16 o0/o1 decode GB/s, batched vs per-block launch; p99 block latency"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_rans_decode_mixed_10k.json")
+SECTIONS = ("c4", "configs", "serve", "cpu")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def events(n):
+    import torch
+    return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+
+
+def slices_to_host(t, offs, lens):
+    """Copy device byte ranges to one compact host array (+ new offsets)."""
+    lens = np.asarray(lens, dtype=np.uint64)
+    o2 = np.zeros(len(lens), dtype=np.uint64)
+    if len(lens):
+        o2[1:] = np.cumsum(lens)[:-1]
+    out = np.empty(int(lens.sum()) + 64, dtype=np.uint8)
+    for a, b, n in zip(offs, o2, lens):
+        out[int(b):int(b) + int(n)] = t[int(a):int(a) + int(n)].cpu().numpy()
+    return out, o2
+
+
+def encode_corpus(coder, corpus, dev, reps=2):
+    """Batched device encode; returns (BatchResult, comp_len, best seconds)."""
+    import torch
+    arena = torch.from_numpy(corpus.data).to(dev)
+    stream = torch.cuda.current_stream()
+    enc = coder.encode_batch(arena, corpus.offs, corpus.sizes, corpus.orders)
+    torch.cuda.synchronize()
+    assert int((enc.status != 0).sum()) == 0, f"{corpus.name}: device encode failed"
+    times = []
+    for _ in range(reps):
+        (a, b), = events(1)
+        a.record(stream)
+        enc = coder.encode_batch(arena, corpus.offs, corpus.sizes, corpus.orders, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) / 1e3)
+    return arena, enc, enc.out_len.cpu().numpy().astype(np.uint64), min(times)
+
+
+def check_sample(oracle, corpus, enc, comp_len, sample):
+    for j in sample:
+        o, n = int(enc.dst_off[j]), int(comp_len[j])
+        got = enc.dst[o:o + n].cpu().numpy().tobytes()
+        ref = oracle.encode(corpus.block(j), int(corpus.orders[j]))
+        assert got == ref, f"{corpus.name} block {j}: device encode differs from the oracle"
+
+
+def perblock(coder, src, src_off, src_len, dst, dst_off, caps, idx, decode=True, orders=None):
+    """The plain form: one launch per block, back to back on one stream."""
+    import torch
+    from fqxv_amd import _lib
+    stream = torch.cuda.current_stream()
+    st1 = torch.zeros(1, dtype=torch.int32, device=src.device)
+    ol1 = torch.zeros(1, dtype=torch.int64, device=src.device)
+    jobs = [_lib.Job(int(src_off[j]), int(src_len[j]), int(dst_off[j]), int(caps[j]), 2**64 - 1,
+                     0 if orders is None else int(orders[j]), 0) for j in idx]
+    fn = coder.decode_one_device if decode else coder.encode_one_device
+    for jb in jobs[:2]:
+        fn(src, dst, jb, st1, ol1, stream=stream)
+    torch.cuda.synchronize()
+    ev = events(len(jobs))
+    t0 = time.perf_counter()
+    for (a, b), jb in zip(ev, jobs):
+        a.record(stream)
+        fn(src, dst, jb, st1, ol1, stream=stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert int(st1.item()) == 0
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    return wall, lat
+
+
+def batched_decode_time(coder, enc, comp_len, sizes, dst, reps=3):
+    import torch
+    stream = torch.cuda.current_stream()
+    dst_off = np.zeros_like(sizes)
+    dst_off[1:] = np.cumsum(sizes)[:-1]
+    jobs = coder.jobs_array(enc.dst_off, comp_len, dst_off, sizes)
+    g = coder.graph("decode", enc.dst, dst, jobs, coder.lpt_order(sizes))
+    g.launch(stream)
+    ev = events(reps)
+    for a, b in ev:
+        a.record(stream)
+        g.launch(stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    assert int((g.status != 0).sum()) == 0
+    g.close()
+    return min(a.elapsed_time(b) for a, b in ev) / 1e3
+
+
+# --------------------------------------------------------------------------- C4
+def bench_c4(args, coder, dev, oracle):
+    import torch
+    from fqxv_amd import synth, tune
+    t0 = time.time()
+    corpus = synth.workload(args.workload, blocks=args.blocks or None)
+    nb, n_total = len(corpus.sizes), corpus.total
+    log(f"[c4] generated {args.workload}: {nb} blocks, {n_total / 1e9:.3f} GB in {time.time() - t0:.1f}s")
+    arena, enc, comp_len, t_enc = encode_corpus(coder, corpus, dev)
+    c_total = int(comp_len.sum())
+    log(f"[c4] batched encode {n_total / t_enc / 1e9:.1f} GB/s, ratio {n_total / c_total:.3f}")
+
+    rng = np.random.default_rng(0)
+    sample = sorted(set(rng.choice(nb, min(24, nb), replace=False).tolist()) | {int(np.argmax(corpus.sizes))})
+    check_sample(oracle, corpus, enc, comp_len, sample)
+    dec = coder.decode_batch(enc.dst, enc.dst_off, comp_len, corpus.sizes)
+    torch.cuda.synchronize()
+    assert int((dec.status != 0).sum()) == 0, "device decode failed"
+    assert torch.equal(dec.dst[:n_total], arena[:n_total]), "device round trip differs"
+    log(f"[c4] parity ok ({len(sample)} blocks vs oracle, all {nb} round-tripped)")
+    eager_ms = batched_decode_eager_ms(coder, enc, comp_len, corpus.sizes)
+
+    out = dec.dst
+    jobs_np = coder.jobs_array(enc.dst_off, comp_len, dec.dst_off, corpus.sizes)
+    if args.grid:
+        coder.set_grid(args.grid)
+        tuned = tune.Tuned(args.grid, args.policy, float("nan"), [])
+    else:
+        tuned = tune.autotune(coder, "decode", enc.dst, out, jobs_np, corpus.sizes, corpus.orders,
+                              use_cache=False, log=log)
+    log(f"[c4] tuned grid={tuned.grid} policy={tuned.policy}")
+    g = coder.graph("decode", enc.dst, out, jobs_np, tuned.order(coder, corpus.sizes, corpus.orders))
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        g.launch(stream)
+    torch.cuda.synchronize()
+    ev = events(args.steps)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        g.launch(stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kern = [a.elapsed_time(b) / 1e3 for a, b in ev]
+    assert int((g.status != 0).sum()) == 0
+    assert torch.equal(out[:n_total], arena[:n_total])
+    t_step = wall / args.steps
+    value = n_total / t_step / 1e9
+    t_kern = float(np.mean(kern))
+    log(f"[c4] graph-replayed batched decode {value:.1f} GB/s ({t_step * 1e3:.2f} ms/step, "
+        f"kernel {t_kern * 1e3:.2f} ms)")
+
+    # Latency floor: one wave decodes a block as ceil(n/32) dependent rounds,
+    # so the largest order-1 block bounds the makespan.
+    big1 = [j for j in range(nb) if corpus.orders[j] == 1]
+    jbig = max(big1, key=lambda j: corpus.sizes[j]) if big1 else int(np.argmax(corpus.sizes))
+    _, lat_big = perblock(coder, enc.dst, enc.dst_off, comp_len, out, dec.dst_off, corpus.sizes, [jbig])
+
+    pb = None
+    if not args.no_perblock:
+        idx = list(range(0, nb, args.perblock_every))
+        wall_pb, lat = perblock(coder, enc.dst, enc.dst_off, comp_len, out, dec.dst_off, corpus.sizes, idx)
+        sb = int(corpus.sizes[idx].sum())
+        pb_gbs = sb / wall_pb / 1e9
+        pb = {"value": round(pb_gbs, 4), "unit": "GB/s",
+              "form": "one kernel launch per block, back to back on one stream",
+              "sample": f"every {args.perblock_every}th block ({len(idx)} launches, {sb / 1e9:.3f} GB)",
+              "speedup_batched": round(value / pb_gbs, 1),
+              "p50_block_ms": round(float(np.percentile(lat, 50)), 3),
+              "p99_block_ms": round(float(np.percentile(lat, 99)), 3)}
+        log(f"[c4] per-block {pb_gbs:.3f} GB/s, batched speedup {value / pb_gbs:.1f}x, "
+            f"p50/p99 {pb['p50_block_ms']}/{pb['p99_block_ms']} ms")
+        torch.cuda.synchronize()
+        assert torch.equal(out[:n_total], arena[:n_total])
+
+    # CPU-baseline sample (host copies of a subset of the streams).
+    cpu_idx = np.arange(0, nb, args.cpu_every)
+    host_c, host_off = slices_to_host(enc.dst, enc.dst_off[cpu_idx], comp_len[cpu_idx])
+    res = {
+        "corpus": corpus, "value": value, "t_step": t_step, "t_kern": t_kern, "n_total": n_total,
+        "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "perblock": pb,
+        "tuned": tuned, "eager_ms": eager_ms, "graph_kernel_ms": t_kern * 1e3,
+        "largest_o1": {"block": int(jbig), "n": int(corpus.sizes[jbig]), "ms": round(float(lat_big[0]), 3),
+                       "rounds": int((int(corpus.sizes[jbig]) + 31) // 32),
+                       "ns_per_round": round(float(lat_big[0]) * 1e6 / ((int(corpus.sizes[jbig]) + 31) // 32),
+                                             1)},
+        "cpu_sample": (host_c, host_off, comp_len[cpu_idx], corpus.sizes[cpu_idx], corpus.orders[cpu_idx]),
+    }
+    g.close()
+    del arena, enc, dec, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def batched_decode_eager_ms(coder, enc, comp_len, sizes, reps=2):
+    """The batched call launched eagerly (no graph), for comparison."""
+    import torch
+    stream = torch.cuda.current_stream()
+    d = coder.decode_batch(enc.dst, enc.dst_off, comp_len, sizes, stream=stream)
+    ev = events(reps)
+    for a, b in ev:
+        a.record(stream)
+        d = coder.decode_batch(enc.dst, enc.dst_off, comp_len, sizes, stream=stream, dst=d.dst)
+        b.record(stream)
+    torch.cuda.synchronize()
+    assert int((d.status != 0).sum()) == 0
+    ms = min(a.elapsed_time(b) for a, b in ev)
+    del d
+    return round(ms, 3)
+
+
+# ------------------------------------------------------------------ C1, C2, C3
+def bench_config(coder, dev, oracle, name, check):
+    import torch
+    from fqxv_amd import synth
+    corpus = synth.workload(name)
+    nb, n_total = len(corpus.sizes), corpus.total
+    arena, enc, comp_len, t_enc_b = encode_corpus(coder, corpus, dev)
+    check_sample(oracle, corpus, enc, comp_len, check)
+    c_total = int(comp_len.sum())
+    idx = list(range(nb))
+    # plain form: encode then decode, one launch per block
+    caps = np.array([coder.encode_bound(int(n), int(o)) for n, o in zip(corpus.sizes, corpus.orders)],
+                    dtype=np.uint64)
+    e_off = np.zeros_like(caps)
+    e_off[1:] = np.cumsum(caps)[:-1]
+    edst = torch.empty(int(caps.sum()) + 64, dtype=torch.uint8, device=dev)
+    w_enc, lat_enc = perblock(coder, arena, corpus.offs, corpus.sizes, edst, e_off, caps, idx, decode=False,
+                              orders=corpus.orders)
+    for j in check[:2]:
+        o = int(e_off[j])
+        assert edst[o:o + int(comp_len[j])].cpu().numpy().tobytes() == oracle.encode(
+            corpus.block(j), int(corpus.orders[j]))
+    del edst
+    dst = torch.empty(n_total + 64, dtype=torch.uint8, device=dev)
+    d_off = corpus.offs
+    w_dec, lat_dec = perblock(coder, enc.dst, enc.dst_off, comp_len, dst, d_off, corpus.sizes, idx)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:n_total], arena[:n_total]), f"{name}: per-block round trip differs"
+    dst.zero_()
+    t_b = batched_decode_time(coder, enc, comp_len, corpus.sizes, dst)
+    assert torch.equal(dst[:n_total], arena[:n_total]), f"{name}: batched round trip differs"
+    n0 = int(corpus.sizes[0])
+    rounds = (n0 + 31) // 32
+    r = {"blocks": nb, "uncompressed_bytes": n_total, "compressed_bytes": c_total,
+         "order": int(corpus.orders[0]), "bits_per_symbol": round(8 * c_total / n_total, 3),
+         "perblock_encode_gbs": round(n_total / w_enc / 1e9, 3),
+         "perblock_decode_gbs": round(n_total / w_dec / 1e9, 3),
+         "batched_encode_gbs": round(n_total / t_enc_b / 1e9, 3),
+         "batched_decode_gbs": round(n_total / t_b / 1e9, 3),
+         "block_decode_ms_p50": round(float(np.percentile(lat_dec, 50)), 3),
+         "block_decode_ms_p99": round(float(np.percentile(lat_dec, 99)), 3),
+         "block_encode_ms_p50": round(float(np.percentile(lat_enc, 50)), 3),
+         "decode_ns_per_round": round(float(np.median(lat_dec)) * 1e6 / rounds, 1),
+         "rounds_per_block": rounds,
+         "decode_roofline_frac": round((n_total + c_total) / t_b / 1e9 / HBM_PEAK_GBS, 5)}
+    log(f"[{name}] " + ", ".join(f"{k}={v}" for k, v in r.items()))
+    state = (corpus, arena, enc, comp_len)
+    return r, state
+
+
+# ------------------------------------------------------------------------ C5
+def bench_serve(args, coder, dev, state):
+    import torch
+    from fqxv_amd.serve import Server, poisson_requests, run_load, summarize
+    corpus, arena, enc, comp_len = state
+    nb = len(corpus.sizes)
+    blk = int(corpus.sizes[0])
+    # batched peak on this corpus: one launch over R copies of every block
+    reps = max(1, args.peak_blocks // nb)
+    k = nb * reps
+    src_off = np.tile(enc.dst_off, reps)
+    src_len = np.tile(comp_len, reps)
+    sizes = np.tile(corpus.sizes, reps)
+    d_off = np.arange(k, dtype=np.uint64) * np.uint64(blk)
+    dst = torch.empty(k * blk + 64, dtype=torch.uint8, device=dev)
+    jobs = coder.jobs_array(src_off, src_len, d_off, sizes)
+    g = coder.graph("decode", enc.dst, dst, jobs, None)
+    g.launch()
+    stream = torch.cuda.current_stream()
+    ev = events(2)
+    for a, b in ev:
+        a.record(stream)
+        g.launch(stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    assert int((g.status != 0).sum()) == 0
+    assert torch.equal(dst[(k - 1) * blk:k * blk], arena[(nb - 1) * blk:nb * blk])
+    t_peak = min(a.elapsed_time(b) for a, b in ev) / 1e3
+    peak = k * blk / t_peak
+    g.close()
+    del dst
+    torch.cuda.empty_cache()
+    log(f"[serve] batched peak {peak / 1e9:.2f} GB/s ({k} blocks in {t_peak * 1e3:.1f} ms)")
+
+    budget = args.serve_budget_gib << 30
+    srv = Server(enc.dst, enc.dst_off, comp_len, corpus.sizes, budget=budget, slots=args.serve_slots,
+                 max_batch=4096)
+    mean_req = 32.5 * blk
+    points = []
+    bad = []
+    for load in args.serve_loads:
+        rate = load * peak / mean_req
+        arrivals, reqs = poisson_requests(5, args.serve_requests, nb, rate, kmax=64)
+
+        def verify(i, c, reqs=reqs):
+            if i % 8:
+                return
+            for t, b in zip(srv.output(c, reqs[i]), reqs[i]):
+                if not torch.equal(t, arena[int(corpus.offs[b]):int(corpus.offs[b]) + blk]):
+                    bad.append((load, i, int(b)))
+
+        recs = run_load(srv, arrivals, reqs, on_done=verify)
+        s = summarize(recs, lambda i, reqs=reqs: len(reqs[i]) * blk)
+        s.update({"offered_load": load, "offered_gbs": round(rate * mean_req / 1e9, 3),
+                  "max_in_flight": srv.max_in_flight, "max_queued": srv.max_queued})
+        points.append(s)
+        log(f"[serve] load {load:.0%}: p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms, {s['gbs']} GB/s, "
+            f"in flight <= {srv.max_in_flight}, queued <= {srv.max_queued}")
+    srv.close()
+    assert not bad, f"serving outputs differ: {bad[:5]}"
+    assert all(p["errors"] == 0 for p in points)
+    return {"corpus": "markov_64x4mib, resident compressed in HBM", "batched_peak_gbs": round(peak / 1e9, 3),
+            "requests_per_point": args.serve_requests, "request_blocks": "uniform 1-64 distinct ids, seed 5",
+            "budget_gib": args.serve_budget_gib, "slots": args.serve_slots, "points": points,
+            "verified": "every 8th request's blocks compared on device with the original bytes"}
+
+
+# ------------------------------------------------------------------------ CPU
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_cpu(args, oracle, sample, workload):
+    host_c, host_off, lens, sizes, orders = sample
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+
+    def run(idx, nthreads, min_s):
+        nbytes, t, passes = 0, 0.0, 0
+        while t < min_s:
+            t0 = time.perf_counter()
+            _, _, ol, st = oracle.decode_many(host_c, host_off[idx], lens[idx], sizes[idx], nthreads)
+            t += time.perf_counter() - t0
+            assert np.all(st == 0)
+            nbytes += int(sizes[idx].sum())
+            passes += 1
+        return nbytes / t / 1e9, passes, t
+
+    allidx = np.arange(len(sizes))
+    gbs_all, p_all, t_all = run(allidx, threads, args.cpu_seconds)
+    sub = allidx[::10]
+    gbs_1, p_1, t_1 = run(sub, 1, args.cpu_seconds / 2)
+    log(f"[cpu] oracle decode {gbs_all:.3f} GB/s on {threads} threads, {gbs_1:.4f} GB/s on 1 thread")
+    return {"value": round(gbs_all, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{len(allidx)} blocks of {workload} (every {args.cpu_every}th, "
+                      f"{int(sizes.sum()) / 1e9:.3f} GB, order-1 {int(orders.sum())}), decoded {p_all}x "
+                      f"block-parallel over {threads} threads in {t_all:.1f}s",
+            "single_thread": {"value": round(gbs_1, 4), "unit": "GB/s",
+                              "sample": f"every 10th block of that sample ({len(sub)} blocks), {p_1}x, "
+                                        f"{t_1:.1f}s"},
+            "host_cpu": cpu_model(), "nproc": os.cpu_count()}
 
 
 def main():
@@ -46,11 +422,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="mixed_10k")
     ap.add_argument("--blocks", type=int, default=0, help="truncate the workload (0 = all)")
+    ap.add_argument("--sections", default=",".join(SECTIONS))
+    ap.add_argument("--grid", type=int, default=0, help="skip the autotuner: persistent grid")
+    ap.add_argument("--policy", default="lpt-cost", help="with --grid: queue order policy")
     ap.add_argument("--perblock-every", type=int, default=20)
+    ap.add_argument("--no-perblock", action="store_true")
     ap.add_argument("--cpu-every", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-perblock", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--peak-blocks", type=int, default=2048)
+    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.1, 0.25, 0.5, 0.75, 1.0, 1.2])
+    ap.add_argument("--serve-budget-gib", type=int, default=32)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 1 GiB read + 1 GiB write kernel (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -58,159 +444,75 @@ def main():
               "multi-GPU form (static LPT sharding of the block list, no collectives) is "
               "not part of this build; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    sections = set(args.sections.split(","))
 
     import torch
     import fqxv_amd
-    from fqxv_amd import synth
+    from oracle import oracle
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda", 0)
     coder = fqxv_amd.Coder(0)
 
-    t0 = time.time()
-    corpus = synth.workload(args.workload, blocks=args.blocks or None)
-    n_blocks = len(corpus.sizes)
-    n_total = corpus.total
-    log(f"[bench] generated {args.workload}: {n_blocks} blocks, {n_total / 1e9:.3f} GB "
-        f"in {time.time() - t0:.1f}s")
-    arena = torch.from_numpy(corpus.data).to(dev)
-    torch.cuda.synchronize()
-
-    # ---- encode (batched, one persistent launch) ---------------------------
-    stream = torch.cuda.current_stream()
-    enc = coder.encode_batch(arena, corpus.offs, corpus.sizes, corpus.orders)
-    torch.cuda.synchronize()
-    assert int((enc.status != 0).sum()) == 0, "device encode failed"
-    enc_times = []
-    for _ in range(2):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        enc = coder.encode_batch(arena, corpus.offs, corpus.sizes, corpus.orders, stream=stream)
-        e1.record(stream)
+    if args.copy_calib:
+        # Known traffic for counter calibration: an elementwise kernel that
+        # reads 1 GiB and writes 1 GiB (fill_ writes 1 GiB).
+        a = torch.empty(1 << 28, dtype=torch.int32, device=dev).fill_(1)
+        b = torch.empty_like(a)
+        torch.add(a, 1, out=b)
         torch.cuda.synchronize()
-        enc_times.append(e0.elapsed_time(e1) / 1e3)
-    comp_len = enc.out_len.cpu().numpy().astype(np.uint64)
-    c_total = int(comp_len.sum())
-    log(f"[bench] encode {n_total / min(enc_times) / 1e9:.1f} GB/s, ratio {n_total / c_total:.3f}")
+        del a, b
 
-    # ---- parity before timing ---------------------------------------------
-    from oracle import oracle
-    host_c = enc.dst.cpu().numpy()
-    rng = np.random.default_rng(0)
-    sample = sorted(set(rng.choice(n_blocks, min(24, n_blocks), replace=False).tolist()) |
-                    {int(np.argmax(corpus.sizes))})
-    for j in sample:
-        got = host_c[int(enc.dst_off[j]):int(enc.dst_off[j]) + int(comp_len[j])].tobytes()
-        ref = oracle.encode(corpus.block(j), int(corpus.orders[j]))
-        assert got == ref, f"block {j}: device encode differs from the oracle"
-    dec = coder.decode_batch(enc.dst, enc.dst_off, comp_len, corpus.sizes)
-    torch.cuda.synchronize()
-    assert int((dec.status != 0).sum()) == 0, "device decode failed"
-    assert torch.equal(dec.dst[:n_total], arena[:n_total]), "device round trip differs"
-    log(f"[bench] parity ok ({len(sample)} blocks vs oracle, all {n_blocks} round-tripped)")
+    c4 = bench_c4(args, coder, dev, oracle)
+    configs = {}
+    serving = None
+    if "configs" in sections or "serve" in sections:
+        coder.set_grid(0)
+        if "configs" in sections:
+            r1, _ = bench_config(coder, dev, oracle, "binned_1mib", [0])
+            r2, _ = bench_config(coder, dev, oracle, "normal_64x1mib", [0, 21, 63])
+            configs = {"binned_1mib": r1, "normal_64x1mib": r2}
+        r3, st3 = bench_config(coder, dev, oracle, "markov_64x4mib", [0, 63])
+        if "configs" in sections:
+            configs["markov_64x4mib"] = r3
+        if "serve" in sections:
+            serving = bench_serve(args, coder, dev, st3)
+        del st3
+    cpu = bench_cpu(args, oracle, c4["cpu_sample"], args.workload) if "cpu" in sections else None
 
-    # ---- batched decode: the timed steps -----------------------------------
-    order_np = coder.lpt_order(corpus.sizes)
-    jobs_np = coder.jobs_array(enc.dst_off, comp_len, dec.dst_off, corpus.sizes)
-    import ctypes
-    jobs_t = torch.from_numpy(jobs_np.view(np.uint8).copy()).to(dev)
-    ord_t = torch.from_numpy(order_np.view(np.int32).copy()).to(dev)
-    status = torch.zeros(n_blocks, dtype=torch.int32, device=dev)
-    olen = torch.zeros(n_blocks, dtype=torch.int64, device=dev)
-    out = dec.dst
-    L = coder._L
-
-    def launch():
-        rc = L.fqxv_rans_decode_batch(coder._h, ctypes.c_void_p(enc.dst.data_ptr()),
-                                      ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(jobs_t.data_ptr()),
-                                      n_blocks, ctypes.c_void_p(ord_t.data_ptr()),
-                                      ctypes.c_void_p(status.data_ptr()), ctypes.c_void_p(olen.data_ptr()),
-                                      ctypes.c_void_p(stream.cuda_stream))
-        assert rc == 0
-
-    for _ in range(args.warmup):
-        launch()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for a, b in evs:
-        a.record(stream)
-        launch()
-        b.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    kern = [a.elapsed_time(b) / 1e3 for a, b in evs]
-    assert int((status != 0).sum()) == 0
-    assert torch.equal(out[:n_total], arena[:n_total])
-    t_step = wall / args.steps
-    value = n_total / t_step / 1e9
-    t_kern = float(np.mean(kern))
-    log(f"[bench] batched decode {value:.1f} GB/s ({t_step * 1e3:.2f} ms/step, kernel {t_kern * 1e3:.2f} ms)")
-
-    # ---- per-block launches (plain form) on a sample ------------------------
-    perblock = None
-    if not args.no_perblock:
-        from fqxv_amd import _lib
-        idx = list(range(0, n_blocks, args.perblock_every))
-        st1 = torch.zeros(1, dtype=torch.int32, device=dev)
-        ol1 = torch.zeros(1, dtype=torch.int64, device=dev)
-        jobs = [_lib.Job(int(enc.dst_off[j]), int(comp_len[j]), int(dec.dst_off[j]), int(corpus.sizes[j]),
-                         2**64 - 1, 0, 0) for j in idx]
-        for jb in jobs[:3]:
-            coder.decode_one_device(enc.dst, out, jb, st1, ol1, stream=stream)
-        torch.cuda.synchronize()
-        pe = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in idx]
-        tp0 = time.perf_counter()
-        for (a, b), jb in zip(pe, jobs):
-            a.record(stream)
-            coder.decode_one_device(enc.dst, out, jb, st1, ol1, stream=stream)
-            b.record(stream)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - tp0
-        lat = np.array([a.elapsed_time(b) for a, b in pe])
-        sb = int(corpus.sizes[idx].sum())
-        pb_gbs = sb / tp / 1e9
-        perblock = {"value": round(pb_gbs, 3), "unit": "GB/s", "sample": f"every {args.perblock_every}th block "
-                    f"({len(idx)} launches, {sb / 1e9:.3f} GB)", "speedup_batched": round(value / pb_gbs, 2),
-                    "p50_block_ms": round(float(np.percentile(lat, 50)), 3),
-                    "p99_block_ms": round(float(np.percentile(lat, 99)), 3)}
-        log(f"[bench] per-block {pb_gbs:.2f} GB/s, batched speedup {value / pb_gbs:.1f}x, "
-            f"p99 {perblock['p99_block_ms']} ms")
-
-    # ---- CPU baseline: the oracle on a sample ------------------------------
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        idx = np.arange(0, n_blocks, args.cpu_every)
-        tc0 = time.perf_counter()
-        dst, doff, ol, stc = oracle.decode_many(host_c, enc.dst_off[idx], comp_len[idx], corpus.sizes[idx],
-                                                threads)
-        tc = time.perf_counter() - tc0
-        assert np.all(stc == 0)
-        sbytes = int(corpus.sizes[idx].sum())
-        cpu = {"value": round(sbytes / tc / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-               "sample": f"every {args.cpu_every}th block of {args.workload} ({len(idx)} blocks, "
-                         f"{sbytes / 1e9:.3f} GB), block-parallel over {threads} threads, {tc:.1f}s"}
-        log(f"[bench] cpu oracle {cpu['value']} GB/s on {threads} threads")
-
+    n_total, c_total, t_kern = c4["n_total"], c4["c_total"], c4["t_kern"]
     achieved = (n_total + c_total) / t_kern / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tf = json.load(f)
+        if tf.get("uncompressed_bytes") == n_total and tf.get("compressed_bytes") == c_total:
+            traffic = tf["hbm_bytes_per_launch"]
+    corpus = c4["corpus"]
+    tuned = c4["tuned"]
     result = {
-        "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3),
+        "metric": METRIC, "value": round(c4["value"], 3), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(c4["t_step"] * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n_blocks} blocks, log-uniform 4 KiB-4 MiB, "
+        "data": "synthetic (host generator, fixed seeds; byte-identical to the oracle on a sample, "
+                "device round trip on every block)",
+        "config": {"workload": f"{args.workload}: {c4['nb']} blocks, log-uniform 4 KiB-4 MiB, "
                                f"order-0/1 50/50, Markov quality / DNA / names, seed 4",
-                   "blocks": n_blocks, "uncompressed_bytes": n_total, "compressed_bytes": c_total,
-                   "order1_blocks": int(corpus.orders.sum())},
+                   "blocks": c4["nb"], "uncompressed_bytes": n_total, "compressed_bytes": c_total,
+                   "order1_blocks": int(corpus.orders.sum()),
+                   "launch": "hipGraph replay of one persistent-kernel launch over all blocks",
+                   "grid": tuned.grid, "queue_policy": tuned.policy},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": "k_rans_decode", "kernel_ms": round(t_kern * 1e3, 3)},
         "cpu_baseline": cpu,
-        "perblock": perblock,
-        "encode": {"value": round(n_total / min(enc_times) / 1e9, 3), "unit": "GB/s",
-                   "form": "batched, one persistent launch"},
+        "perblock": c4["perblock"],
+        "batched_eager_ms": c4["eager_ms"],
+        "latency_floor": c4["largest_o1"],
+        "tuner": {"grid": tuned.grid, "policy": tuned.policy, "table": tuned.table},
+        "encode": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s", "form": "batched, one persistent launch"},
+        "configs": configs,
+        "serving": serving,
     }
     print(json.dumps(result), flush=True)
 

@@ -176,7 +176,7 @@ int fqxv_rans_ctx_create(int device, fqxv_rans_ctx **out)
     hipDeviceProp_t prop;
     int dec_occ = 0, enc_occ = 0;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || fqxv_decode_occupancy(&dec_occ) != hipSuccess ||
-        fqxv_encode_occupancy(&enc_occ) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        fqxv_encode_occupancy(&enc_occ) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void **)&c->counters, N_COUNTERS * sizeof(u32)) != hipSuccess ||
         hipMalloc((void **)&c->d_status, sizeof(i32)) != hipSuccess ||
         hipMalloc((void **)&c->d_out_len, sizeof(u64)) != hipSuccess) {
@@ -394,7 +394,7 @@ int fqxv_rans_graph_create(fqxv_rans_ctx *c, int direction, const uint8_t *d_src
     if (!g) return FQXV_RANS_ERR_DEVICE;
     g->ctx = c;
     hipStream_t cs = nullptr;
-    bool ok = hipMalloc((void **)&g->counter, sizeof(u32)) == hipSuccess && hipStreamCreate(&cs) == hipSuccess &&
+    bool ok = hipMalloc((void **)&g->counter, sizeof(u32)) == hipSuccess && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess &&
               hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
         const hipError_t e = enc ? fqxv_launch_encode_batch(d_src, d_dst, d_jobs, count, d_order, g->counter,

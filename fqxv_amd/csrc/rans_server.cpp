@@ -270,7 +270,7 @@ int fqxv_rans_server_create(int device, const uint8_t *d_archive, const fqxv_ran
     s->slots.resize(slots);
     for (Slot &sl : s->slots) {
         if (!ok) break;
-        ok = fqxv_rans_ctx_create(device, &sl.ctx) == FQXV_RANS_OK && hipStreamCreate(&sl.stream) == hipSuccess &&
+        ok = fqxv_rans_ctx_create(device, &sl.ctx) == FQXV_RANS_OK && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
              hipHostMalloc((void **)&sl.h_jobs, max_batch * sizeof(fqxv_rans_job), 0) == hipSuccess &&
              hipHostMalloc((void **)&sl.h_order, max_batch * sizeof(u32), 0) == hipSuccess &&

@@ -117,7 +117,9 @@ int ensure_buf(u8 **p, u64 *cap, u64 need)
     return FQXV_RANS_OK;
 }
 
-hipStream_t pick_stream(fqxv_rans_ctx *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
+// The caller's stream; NULL is the HIP null stream (not the context's
+// stream, which only the host-buffer calls use).
+hipStream_t pick_stream(fqxv_rans_ctx *, void *stream) { return (hipStream_t)stream; }
 
 u32 *next_counter(fqxv_rans_ctx *c)
 {
@@ -176,7 +178,7 @@ int fqxv_rans_ctx_create(int device, fqxv_rans_ctx **out)
     hipDeviceProp_t prop;
     int dec_occ = 0, enc_occ = 0;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess || fqxv_decode_occupancy(&dec_occ) != hipSuccess ||
-        fqxv_encode_occupancy(&enc_occ) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        fqxv_encode_occupancy(&enc_occ) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipMalloc((void **)&c->counters, N_COUNTERS * sizeof(u32)) != hipSuccess ||
         hipMalloc((void **)&c->d_status, sizeof(i32)) != hipSuccess ||
         hipMalloc((void **)&c->d_out_len, sizeof(u64)) != hipSuccess) {

@@ -82,9 +82,9 @@ size_t fqxv_rans_encode_bound(size_t n, int order);
  * when fewer than 9 bytes are given. */
 int fqxv_rans_peek_len(const uint8_t *src, size_t len, uint64_t *n);
 
-/* Context on HIP device `device` (the library owns a stream and a lazily
- * grown device workspace).  lib.rs has no context: the Rust wrapper keeps one
- * per thread. */
+/* Context on HIP device `device` (the library owns a blocking stream for
+ * the host-buffer calls and a lazily grown device workspace).  lib.rs has no
+ * context: the Rust wrapper keeps one per thread. */
 int fqxv_rans_ctx_create(int device, fqxv_rans_ctx **out);
 void fqxv_rans_ctx_destroy(fqxv_rans_ctx *ctx);
 
@@ -112,8 +112,8 @@ int fqxv_rans_decode_bounded(fqxv_rans_ctx *ctx, const uint8_t *src, size_t len,
  * `d_order` (optional, may be NULL) is a permutation of 0..count-1 giving the
  * order in which waves take jobs (largest first is best; see
  * fqxv_rans_lpt_order).  Per job: d_status[i] = return code,
- * d_out_len[i] = bytes written.  Asynchronous on `stream` (NULL = the
- * context's stream); the call is capturable into a hipGraph. */
+ * d_out_len[i] = bytes written.  Asynchronous on `stream` (a hipStream_t;
+ * NULL is the HIP null stream); the call is capturable into a hipGraph. */
 int fqxv_rans_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_t *d_dst,
                            const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
                            int32_t *d_status, uint64_t *d_out_len, void *stream);

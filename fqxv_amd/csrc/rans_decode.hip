@@ -46,7 +46,6 @@ struct DecShared {
         u32 outst[OUT_BYTES / 4];                     // output staging (decode loop)
         u32 fc[256];                                  // f << 16 | cum (table builds)
     };
-    u32 sbuf[NSTATES];                                // order-1: broadcast slots
     u16 dense[256];                                   // order-1: context byte -> dense id
     u8 ulane[256];                                    // order-1: symbol -> lane in U
 };
@@ -393,6 +392,9 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     }
     t8[TOTFREQ * K + l] = 0;
     __threadfence();
+    // Raw buffer view of t8 (gfx9 resource word 3), bounded to the table.
+    const __amdgpu_buffer_rsrc_t t8rs =
+        __builtin_amdgcn_make_buffer_rsrc(t8, (short)0, (int)(TOTFREQ * K + 64), 0x00020000);
 
     WordRing wr;
     wr.init(s + hdr, end);
@@ -409,11 +411,13 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     u64 r = 0;
     while (r < rounds) {
         const u32 slot = x & (TOTFREQ - 1);
-        if (lo) sh.sbuf[l] = slot * K;
+        const u32 rowoff = slot * K;
         u32 cand[NSTATES];
-        // Row offsets are broadcast through LDS; lane l reads column l.
+        // Each slot's row offset is the scalar offset of a raw buffer load
+        // (no LDS round trip); lane l reads column l.
 #pragma unroll
-        for (u32 j = 0; j < NSTATES; j++) cand[j] = t8[sh.sbuf[j] + l];
+        for (u32 j = 0; j < NSTATES; j++)
+            cand[j] = __builtin_amdgcn_raw_buffer_load_b8(t8rs, l, __builtin_amdgcn_readlane(rowoff, j), 0);
         if (r < full && prev < K && pos + NSTATES <= avail) {
             // Fast round, no branch inside: a full round, a valid first
             // context, no truncation possible and the ring holds every word
@@ -428,9 +432,11 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
                 c = __builtin_amdgcn_readlane(cand[j], c);
                 sv = writelane(c, j, sv);
             }
-            // Position j's context is position j-1's symbol (prev for j = 0).
-            u32 ctxv = __shfl_up(sv, 1);
-            if (l == 0) ctxv = prev;
+            // Position j's context is position j-1's symbol: a one-lane shift
+            // within each 16-lane row (DPP row_shr:1) plus the two row heads.
+            u32 ctxv = (u32)__builtin_amdgcn_update_dpp((int)sv, (int)sv, 0x111, 0xf, 0xf, false);
+            ctxv = writelane(prev, 0, ctxv);
+            ctxv = writelane(__builtin_amdgcn_readlane(sv, 15), 16, ctxv);
             const u32 cv = ctxv | (sv << 8);
             const u32 ent = fc2[ctxv * UMAX + sv];
             const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);

@@ -158,6 +158,7 @@ def bench_c4(args, coder, dev, oracle):
     rng = np.random.default_rng(0)
     sample = sorted(set(rng.choice(nb, min(24, nb), replace=False).tolist()) | {int(np.argmax(corpus.sizes))})
     check_sample(oracle, corpus, enc, comp_len, sample)
+    best = bench_encode_best(coder, corpus, arena, oracle, sample[:6]) if args.encode_best else None
     dec = coder.decode_batch(enc.dst, enc.dst_off, comp_len, corpus.sizes)
     torch.cuda.synchronize()
     assert int((dec.status != 0).sum()) == 0, "device decode failed"
@@ -224,6 +225,7 @@ def bench_c4(args, coder, dev, oracle):
     cpu_idx = np.arange(0, nb, args.cpu_every)
     host_c, host_off = slices_to_host(enc.dst, enc.dst_off[cpu_idx], comp_len[cpu_idx])
     res = {
+        "encode_best": best,
         "corpus": corpus, "value": value, "t_step": t_step, "t_kern": t_kern, "n_total": n_total,
         "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "perblock": pb,
         "tuned": tuned, "eager_ms": eager_ms, "graph_kernel_ms": t_kern * 1e3,
@@ -235,6 +237,35 @@ def bench_c4(args, coder, dev, oracle):
     }
     g.close()
     del arena, enc, dec, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_encode_best(coder, corpus, arena, oracle, sample):
+    """Both orders per block, the shorter kept (ties to order-0, as the
+    tokenizer does): batched GB/s, and how many blocks order-1 wins."""
+    import torch
+    stream = torch.cuda.current_stream()
+    r = coder.encode_best_batch(arena, corpus.offs, corpus.sizes, 0, stream=stream)
+    torch.cuda.synchronize()
+    assert int((r.status != 0).sum()) == 0
+    (a, b), = events(1)
+    a.record(stream)
+    r = coder.encode_best_batch(arena, corpus.offs, corpus.sizes, 0, stream=stream)
+    b.record(stream)
+    torch.cuda.synchronize()
+    t = a.elapsed_time(b) / 1e3
+    ol = r.out_len.cpu().numpy()
+    for j in sample:
+        o = int(r.dst_off[j])
+        assert r.dst[o:o + int(ol[j])].cpu().numpy().tobytes() == oracle.encode_best(corpus.block(j), 0), j
+    tags = r.dst[torch.from_numpy(r.dst_off.astype(np.int64)).to(r.dst.device)].cpu().numpy()
+    n = corpus.total
+    res = {"value": round(n / t / 1e9, 3), "unit": "GB/s", "ties": "order-0",
+           "compressed_bytes": int(ol.sum()), "order1_chosen": int(tags.sum()),
+           "checked_vs_oracle": len(sample)}
+    log(f"[c4] batched encode-best {res['value']} GB/s, order-1 kept for {res['order1_chosen']} of {len(ol)} blocks")
+    del r
     torch.cuda.empty_cache()
     return res
 
@@ -427,6 +458,7 @@ def main():
     ap.add_argument("--policy", default="lpt-cost", help="with --grid: queue order policy")
     ap.add_argument("--perblock-every", type=int, default=20)
     ap.add_argument("--no-perblock", action="store_true")
+    ap.add_argument("--encode-best", action=argparse.BooleanOptionalAction, default=True)
     ap.add_argument("--cpu-every", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -511,6 +543,7 @@ def main():
         "latency_floor": c4["largest_o1"],
         "tuner": {"grid": tuned.grid, "policy": tuned.policy, "table": tuned.table},
         "encode": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s", "form": "batched, one persistent launch"},
+        "encode_best": c4["encode_best"],
         "configs": configs,
         "serving": serving,
     }

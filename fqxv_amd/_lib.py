@@ -36,6 +36,7 @@ EXPORTS = [
     "fqxv_rans_ctx_set_grid", "fqxv_rans_server_create", "fqxv_rans_server_destroy",
     "fqxv_rans_server_now", "fqxv_rans_server_submit",
     "fqxv_rans_server_poll", "fqxv_rans_server_take", "fqxv_rans_server_release",
+    "fqxv_rans_encode_best_batch", "fqxv_rans_encode_best_bound",
 ]
 
 
@@ -85,7 +86,9 @@ def load():
         L.fqxv_rans_encode.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz, ctypes.POINTER(sz)]
         L.fqxv_rans_decode.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(sz)]
         L.fqxv_rans_decode_bounded.argtypes = [vp, vp, sz, sz, vp, sz, ctypes.POINTER(sz)]
-        for fn in (L.fqxv_rans_decode_batch, L.fqxv_rans_encode_batch):
+        L.fqxv_rans_encode_best_bound.restype = sz
+        L.fqxv_rans_encode_best_bound.argtypes = [sz]
+        for fn in (L.fqxv_rans_decode_batch, L.fqxv_rans_encode_batch, L.fqxv_rans_encode_best_batch):
             fn.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp, vp]
         for fn in (L.fqxv_rans_decode_one_device, L.fqxv_rans_encode_one_device):
             fn.argtypes = [vp, vp, vp, ctypes.POINTER(Job), vp, vp, vp]

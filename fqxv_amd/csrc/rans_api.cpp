@@ -27,7 +27,7 @@ hipError_t fqxv_launch_decode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32
 hipError_t fqxv_launch_decode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
 hipError_t fqxv_decode_occupancy(int *);
 hipError_t fqxv_launch_encode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32, const u32 *, u32 *, i32 *,
-                                    u64 *, u8 *, u32, hipStream_t);
+                                    u64 *, u8 *, u32, hipStream_t, int);
 hipError_t fqxv_launch_encode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
 hipError_t fqxv_encode_occupancy(int *);
 
@@ -308,10 +308,12 @@ int fqxv_rans_encode(fqxv_rans_ctx *c, const uint8_t *src, size_t n, int order, 
     return FQXV_RANS_OK;
 }
 
+// mode: 0 decode, 1 encode, 2 encode-best.
 static int batch_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
                         uint32_t count, const uint32_t *d_order, int32_t *d_status, uint64_t *d_out_len,
-                        void *stream, bool enc)
+                        void *stream, int mode)
 {
+    const bool enc = mode != 0;
     if (!c) return FQXV_RANS_ERR_INVALID_ARG;
     if (count == 0) return FQXV_RANS_OK;
     if (!d_src || !d_dst || !d_jobs || !d_status || !d_out_len) return FQXV_RANS_ERR_INVALID_ARG;
@@ -323,7 +325,7 @@ static int batch_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, 
     hipStream_t st = pick_stream(c, stream);
     u32 *ctr = next_counter(c);
     const hipError_t e = enc ? fqxv_launch_encode_batch(d_src, d_dst, d_jobs, count, d_order, ctr, d_status,
-                                                        d_out_len, c->ws, grid, st)
+                                                        d_out_len, c->ws, grid, st, mode == 2)
                              : fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, ctr, d_status,
                                                         d_out_len, c->ws, grid, st);
     return e == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
@@ -333,15 +335,24 @@ int fqxv_rans_decode_batch(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_ds
                            uint32_t count, const uint32_t *d_order, int32_t *d_status, uint64_t *d_out_len,
                            void *stream)
 {
-    return batch_common(c, d_src, d_dst, d_jobs, count, d_order, d_status, d_out_len, stream, false);
+    return batch_common(c, d_src, d_dst, d_jobs, count, d_order, d_status, d_out_len, stream, 0);
 }
 
 int fqxv_rans_encode_batch(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
                            uint32_t count, const uint32_t *d_order, int32_t *d_status, uint64_t *d_out_len,
                            void *stream)
 {
-    return batch_common(c, d_src, d_dst, d_jobs, count, d_order, d_status, d_out_len, stream, true);
+    return batch_common(c, d_src, d_dst, d_jobs, count, d_order, d_status, d_out_len, stream, 1);
 }
+
+int fqxv_rans_encode_best_batch(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
+                                uint32_t count, const uint32_t *d_order, int32_t *d_status, uint64_t *d_out_len,
+                                void *stream)
+{
+    return batch_common(c, d_src, d_dst, d_jobs, count, d_order, d_status, d_out_len, stream, 2);
+}
+
+size_t fqxv_rans_encode_best_bound(size_t n) { return fqxv_rans_encode_bound(n, 0) + fqxv_rans_encode_bound(n, 1); }
 
 static int one_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,
                       int32_t *d_status, uint64_t *d_out_len, void *stream, bool enc)
@@ -380,13 +391,13 @@ int fqxv_rans_graph_create(fqxv_rans_ctx *c, int direction, const uint8_t *d_src
                            const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
                            int32_t *d_status, uint64_t *d_out_len, fqxv_rans_graph **out)
 {
-    if (!c || !out || (direction != 0 && direction != 1) || count == 0 || !d_src || !d_dst || !d_jobs ||
+    if (!c || !out || direction < 0 || direction > 2 || count == 0 || !d_src || !d_dst || !d_jobs ||
         !d_status || !d_out_len)
         return FQXV_RANS_ERR_INVALID_ARG;
     *out = nullptr;
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->device);
-    const bool enc = direction == 1;
+    const bool enc = direction != 0;
     const u32 grid = std::min<u32>(count, enc ? c->enc_waves : c->dec_waves);
     // The graph bakes in the workspace address: size it for the largest grid
     // any later call on this context can ask for, so it is never reallocated.
@@ -400,7 +411,7 @@ int fqxv_rans_graph_create(fqxv_rans_ctx *c, int direction, const uint8_t *d_src
               hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
         const hipError_t e = enc ? fqxv_launch_encode_batch(d_src, d_dst, d_jobs, count, d_order, g->counter,
-                                                            d_status, d_out_len, c->ws, grid, cs)
+                                                            d_status, d_out_len, c->ws, grid, cs, direction == 2)
                                  : fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, g->counter,
                                                             d_status, d_out_len, c->ws, grid, cs);
         const hipError_t e2 = hipStreamEndCapture(cs, &g->graph);

@@ -420,12 +420,53 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     return 0;
 }
 
+__device__ __forceinline__ u64 encode_bound_dev(u64 n, int order)
+{
+    const u64 k_max = n < 256 ? n : 256;
+    return n == 0 ? 9 : (order == 0 ? O0_HDR : HDR0 + BITMAP + 512 * k_max + 4 * NSTATES) + 2 * n + 16;
+}
+
+// Both orders, the smaller stream kept (the callers' encode_best:
+// crates/fqxv-tokenizer/src/lib.rs:91-95 and container/reorder.rs:597-607 keep
+// order-0 on a tie, crates/fqxv-lroverlap/src/codec.rs:292-303 keeps order-1).
+// `jb.order` names the order kept on a tie.  Order-0 is written at the front
+// of the region, order-1 behind it; an order-1 winner slides down to the
+// front.  The stream's first byte records the choice.
+__device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, u64 &out_len,
+                               u8 *ws)
+{
+    const int tie = jb.order;
+    if (tie != 0 && tie != 1) return FQXV_RANS_ERR_INVALID_ARG;
+    if (jb.src_len >= (1ull << 32)) return FQXV_RANS_ERR_INVALID_ARG;
+    const u64 b0 = encode_bound_dev(jb.src_len, 0);
+    if (jb.dst_cap < b0 + encode_bound_dev(jb.src_len, 1)) return FQXV_RANS_ERR_DST_TOO_SMALL;
+    fqxv_rans_job j0 = jb, j1 = jb;
+    j0.order = 0;
+    j0.dst_cap = b0;
+    j1.order = 1;
+    j1.dst_off = jb.dst_off + b0;
+    j1.dst_cap = jb.dst_cap - b0;
+    u64 l0 = 0, l1 = 0;
+    int err = encode_job(sh, src, dst, j0, l0, ws);
+    if (err) return err;
+    __syncthreads();
+    err = encode_job(sh, src, dst, j1, l1, ws);
+    if (err) return err;
+    const bool one = tie == 1 ? l1 <= l0 : l1 < l0;
+    if (one) {
+        __threadfence();
+        wave_copy_fwd(dst + jb.dst_off, dst + j1.dst_off, l1);
+    }
+    out_len = one ? l1 : l0;
+    return 0;
+}
+
 } // namespace
 
 __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, u8 *__restrict__ dst,
                                                     const fqxv_rans_job *__restrict__ jobs, u32 count,
                                                     const u32 *__restrict__ order, u32 *counter,
-                                                    i32 *status, u64 *out_len, u8 *ws)
+                                                    i32 *status, u64 *out_len, u8 *ws, int best)
 {
     __shared__ EncShared sh;
     u8 *myws = ws + (u64)blockIdx.x * WS_PER_WAVE;
@@ -437,7 +478,7 @@ __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, 
         const u32 j = order ? order[w] : w;
         const fqxv_rans_job jb = jobs[j];
         u64 ol = 0;
-        const int err = encode_job(sh, src, dst, jb, ol, myws);
+        const int err = best ? encode_best_job(sh, src, dst, jb, ol, myws) : encode_job(sh, src, dst, jb, ol, myws);
         if (lane_id() == 0) {
             status[j] = err;
             out_len[j] = err ? 0 : ol;
@@ -465,11 +506,11 @@ __global__ void k_reset_counter_enc(u32 *counter) { *counter = 0; }
 
 hipError_t fqxv_launch_encode_batch(const u8 *src, u8 *dst, const fqxv_rans_job *jobs, u32 count,
                                     const u32 *order, u32 *counter, i32 *status, u64 *out_len, u8 *ws,
-                                    u32 grid, hipStream_t st)
+                                    u32 grid, hipStream_t st, int best)
 {
     hipLaunchKernelGGL(k_reset_counter_enc, dim3(1), dim3(1), 0, st, counter);
     hipLaunchKernelGGL(k_rans_encode, dim3(grid), dim3(64), 0, st, src, dst, jobs, count, order, counter,
-                       status, out_len, ws);
+                       status, out_len, ws, best);
     return hipGetLastError();
 }
 

@@ -230,6 +230,31 @@ class Coder:
         self._keep = keep
         return BatchResult(dst, dst_off, out_len, status)
 
+    def encode_best_bound(self, n: int) -> int:
+        return int(self._L.fqxv_rans_encode_best_bound(int(n)))
+
+    def encode_best_batch(self, src, src_off, src_len, tie_order=0, lpt: bool = True, stream=None) -> BatchResult:
+        """Encode every block at both orders and keep the smaller stream.
+
+        ``tie_order`` (scalar or per block) is the order kept when both
+        streams have the same length: 0 as the tokenizer and the permutation
+        planes do, 1 as the long-read codec does."""
+        import torch
+        src_off = np.asarray(src_off, dtype=np.uint64)
+        src_len = np.asarray(src_len, dtype=np.uint64)
+        ties = np.broadcast_to(np.asarray(tie_order, dtype=np.int32), src_len.shape).copy()
+        caps = np.array([self.encode_best_bound(int(n)) for n in src_len], dtype=np.uint64)
+        dst_off = np.zeros_like(caps)
+        if len(caps):
+            dst_off[1:] = np.cumsum(caps)[:-1]
+        dst = torch.empty(int(caps.sum()) + 64, dtype=torch.uint8, device=src.device)
+        jobs = self.jobs_array(src_off, src_len, dst_off, caps, order=ties)
+        order_np = self.lpt_order(src_len) if lpt else None
+        status, out_len, keep = self._run_batch(self._L.fqxv_rans_encode_best_batch, src, dst, jobs, order_np,
+                                                stream)
+        self._keep = keep
+        return BatchResult(dst, dst_off, out_len, status)
+
     def decode_batch(self, src, src_off, src_len, out_len, max_len=None, lpt: bool = True,
                      stream=None, dst=None, dst_off=None) -> BatchResult:
         """Decode every stream of a device arena in one persistent launch.
@@ -274,10 +299,10 @@ class Coder:
         _raise(self._L.fqxv_rans_ctx_set_grid(self._h, int(waves)))
 
     def graph(self, direction: str, src, dst, jobs_np: np.ndarray, order_np=None) -> "Graph":
-        """Capture one batched call (``"decode"`` or ``"encode"``) over fixed
+        """Capture one batched call (``"decode"``, ``"encode"`` or ``"encode_best"``) over fixed
         device buffers into a hipGraph; :meth:`Graph.launch` replays it."""
         import torch
-        d = {"decode": 0, "encode": 1}[direction]
+        d = {"decode": 0, "encode": 1, "encode_best": 2}[direction]
         dev = src.device
         jobs = torch.from_numpy(jobs_np.view(np.uint8).copy()).to(dev)
         k = len(jobs_np)

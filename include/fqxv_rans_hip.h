@@ -121,6 +121,18 @@ int fqxv_rans_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_t *d_
                            const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
                            int32_t *d_status, uint64_t *d_out_len, void *stream);
 
+/* Batched encode at both orders, keeping the smaller stream per job: the
+ * callers' encode_best (crates/fqxv-tokenizer/src/lib.rs:91-95,
+ * crates/fqxv/src/container/reorder.rs:597-607 keep order-0 on a tie,
+ * crates/fqxv-lroverlap/src/codec.rs:292-303 keeps order-1), which today
+ * encodes every stream twice on the CPU.  Per job, `order` is the order kept
+ * on a tie and the destination region holds fqxv_rans_encode_best_bound(n)
+ * bytes; the chosen stream starts at dst_off (its first byte is its order). */
+int fqxv_rans_encode_best_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_t *d_dst,
+                                const fqxv_rans_job *d_jobs, uint32_t count, const uint32_t *d_order,
+                                int32_t *d_status, uint64_t *d_out_len, void *stream);
+size_t fqxv_rans_encode_best_bound(size_t n);
+
 /* The plain form: one kernel launch per block (the reference's per-call
  * granularity, lib.rs:111-200), device-resident, asynchronous on `stream`.
  * `job` is a host struct; status and out_len are device pointers. */
@@ -134,9 +146,9 @@ int fqxv_rans_encode_one_device(fqxv_rans_ctx *ctx, const uint8_t *d_src, uint8_
 /* Batched call captured once into a hipGraph and replayed: the work-queue
  * reset kernel and the persistent kernel, arguments fixed at creation (the
  * graph owns its own queue counter).  `direction` is 0 for decode, 1 for
- * encode.  No reference counterpart: the form the container's per-block
- * dispatch (decompress.rs:45-52) takes when the same layout is decoded
- * repeatedly.  A replay must not overlap another call on the same context;
+ * encode, 2 for encode-best.  No reference counterpart: the form the
+ * container's per-block dispatch (decompress.rs:45-52) takes when the same
+ * layout is decoded repeatedly.  A replay must not overlap another call on the same context;
  * the context must outlive its graphs.  The buffers passed at creation are
  * read and written by every replay. */
 typedef struct fqxv_rans_graph fqxv_rans_graph;

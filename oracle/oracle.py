@@ -110,6 +110,18 @@ def encode(data, order: int) -> bytes:
     return dst[: out.value].tobytes()
 
 
+def encode_best(data, tie_order: int = 0) -> bytes:
+    """Both orders, the smaller kept; ``tie_order`` wins a tie.  Follows the
+    callers: crates/fqxv-tokenizer/src/lib.rs:91-95 and
+    crates/fqxv/src/container/reorder.rs:597-607 (order-0 on a tie),
+    crates/fqxv-lroverlap/src/codec.rs:292-303 (order-1 on a tie)."""
+    e0 = encode(data, 0)
+    e1 = encode(data, 1)
+    if tie_order == 1:
+        return e1 if len(e1) <= len(e0) else e0
+    return e1 if len(e1) < len(e0) else e0
+
+
 def decode_ex(data, cap: int | None = None):
     """Return (status, output bytes, consumed stream bytes)."""
     src = _as_u8(data)

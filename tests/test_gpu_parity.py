@@ -220,3 +220,32 @@ def test_batched_decode_writes_only_its_region(coder, oracle):
         assert out[o:o + len(b)].tobytes() == b, j
         assert out[o + len(b)] == 0xEE, f"block {j} wrote past its region"
     assert np.all(out[:3] == 0xEE)
+
+
+def test_encode_best_batch_matches_oracle(coder, oracle):
+    """Both orders per block on the device, the shorter kept, each block's
+    tie rule honoured (empty blocks are ties); decoded back on the device."""
+    import torch
+    from fqxv_amd import synth
+    blocks = [b"", synth.fill(synth.K_MARKOV, 30000, 3).tobytes(), synth.fill(synth.K_DNA, 30000, 4).tobytes(),
+              b"", b"Q" * 5000, synth.fill(synth.K_NAMES, 12345, 5).tobytes(), bytes(range(256)) * 20,
+              synth.fill(synth.K_MARKOV, 1 << 20, 6).tobytes(), b"x"]
+    ties = np.array([0, 0, 1, 1, 0, 1, 0, 1, 1], dtype=np.int32)
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    offs = np.zeros(len(blocks), dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(blocks) + bytes(64), dtype=np.uint8).copy()).cuda()
+    r = coder.encode_best_batch(arena, offs, sizes, ties)
+    torch.cuda.synchronize()
+    assert np.all(r.status.cpu().numpy() == 0)
+    ol = r.out_len.cpu().numpy()
+    host = r.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        got = host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes()
+        assert got == oracle.encode_best(b, int(ties[j])), j
+    d = coder.decode_batch(r.dst, r.dst_off, ol, sizes)
+    torch.cuda.synchronize()
+    assert np.all(d.status.cpu().numpy() == 0)
+    out = d.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        assert out[int(d.dst_off[j]):int(d.dst_off[j]) + len(b)].tobytes() == b, j

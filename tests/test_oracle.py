@@ -332,3 +332,24 @@ def test_mutations_never_crash(oracle):
         if n is not None and n > len(data):
             continue
         oracle.decode_ex(bytes(enc), cap=len(data))
+
+
+def test_encode_best_follows_each_callers_tie_rule(oracle):
+    """encode_best keeps the shorter stream; on a tie the tokenizer and the
+    permutation planes keep order-0 and the long-read codec order-1
+    (crates/fqxv-tokenizer/src/lib.rs:91-95, container/reorder.rs:597-607,
+    crates/fqxv-lroverlap/src/codec.rs:292-303).  The empty input is a tie
+    (both streams are the 9-byte header), so the tag byte shows the rule."""
+    assert oracle.encode_best(b"", 0) == bytes(9)
+    assert oracle.encode_best(b"", 1) == b"\x01" + bytes(8)
+    from fqxv_amd import synth
+    markov = synth.fill(synth.K_MARKOV, 200000, 3).tobytes()
+    dna = synth.fill(synth.K_DNA, 20000, 4).tobytes()
+    for data in (markov, dna, b"A" * 100, bytes(range(256)) * 8):
+        e0, e1 = oracle.encode(data, 0), oracle.encode(data, 1)
+        for tie in (0, 1):
+            got = oracle.encode_best(data, tie)
+            assert len(got) == min(len(e0), len(e1))
+            assert got == (e1 if len(e1) < len(e0) else e0)
+    assert oracle.encode_best(markov, 0)[0] == 1   # context helps Markov qualities
+    assert oracle.encode_best(dna, 0)[0] == 0      # i.i.d. bases: order-1 only adds tables

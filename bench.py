@@ -337,6 +337,41 @@ def bench_config(coder, dev, oracle, name, check):
     return r, state
 
 
+# ------------------------------------------------------------ permutation planes
+def bench_perm(coder, dev, oracle, n=1 << 26):
+    """reorder.rs:597-607 / 1089-1100 transposes on the device: u32 indices
+    <-> four byte planes.  HBM-bound: 8n algorithmic bytes per launch."""
+    import torch
+    from fqxv_amd import perm
+    stream = torch.cuda.current_stream()
+    p = torch.randint(-2**31, 2**31 - 1, (n,), dtype=torch.int32, device=dev)
+    planes = perm.split(p, coder=coder)
+    back = perm.join(planes, n, coder=coder)
+    torch.cuda.synchronize()
+    assert torch.equal(back, p)
+    k = 4096
+    assert np.array_equal(planes[:4 * n].view(4, n)[:, :k].cpu().numpy().T.reshape(-1).view("<u4"),
+                          p[:k].cpu().numpy().view(np.uint32))
+    res = {}
+    for name, fn in (("split", lambda: perm.split(p, planes, coder=coder)),
+                     ("join", lambda: perm.join(planes, n, back, coder=coder))):
+        fn()
+        ev = events(5)
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        t = min(a.elapsed_time(b) for a, b in ev) / 1e3
+        res[name] = {"gbs": round(8 * n / t / 1e9, 1), "ms": round(t * 1e3, 3),
+                     "roofline_frac": round(8 * n / t / 1e9 / HBM_PEAK_GBS, 3)}
+    assert torch.equal(back, p)
+    log(f"[perm] split {res['split']['gbs']} GB/s, join {res['join']['gbs']} GB/s (n = {n})")
+    del p, planes, back
+    torch.cuda.empty_cache()
+    return {"indices": n, "bytes_per_launch": 8 * n, **res}
+
+
 # ------------------------------------------------------------------------ C5
 def bench_serve(args, coder, dev, state):
     import torch
@@ -503,7 +538,7 @@ def main():
         if "configs" in sections:
             r1, _ = bench_config(coder, dev, oracle, "binned_1mib", [0])
             r2, _ = bench_config(coder, dev, oracle, "normal_64x1mib", [0, 21, 63])
-            configs = {"binned_1mib": r1, "normal_64x1mib": r2}
+            configs = {"binned_1mib": r1, "normal_64x1mib": r2, "perm_planes": bench_perm(coder, dev, oracle)}
         r3, st3 = bench_config(coder, dev, oracle, "markov_64x4mib", [0, 63])
         if "configs" in sections:
             configs["markov_64x4mib"] = r3

@@ -30,6 +30,8 @@ hipError_t fqxv_launch_encode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32
                                     u64 *, u8 *, u32, hipStream_t, int);
 hipError_t fqxv_launch_encode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
 hipError_t fqxv_encode_occupancy(int *);
+hipError_t fqxv_launch_perm_split(const u32 *, u64, u8 *, hipStream_t);
+hipError_t fqxv_launch_perm_join(const u8 *, u64, u32 *, hipStream_t);
 
 namespace {
 
@@ -433,6 +435,20 @@ int fqxv_rans_graph_launch(fqxv_rans_graph *g, void *stream)
     DeviceGuard dg(g->ctx->device);
     return hipGraphLaunch(g->exec, pick_stream(g->ctx, stream)) == hipSuccess ? FQXV_RANS_OK
                                                                              : FQXV_RANS_ERR_DEVICE;
+}
+
+int fqxv_rans_perm_split(const uint32_t *d_perm, uint64_t n, uint8_t *d_planes, void *stream)
+{
+    if (n && (!d_perm || !d_planes)) return FQXV_RANS_ERR_INVALID_ARG;
+    return fqxv_launch_perm_split(d_perm, n, d_planes, (hipStream_t)stream) == hipSuccess ? FQXV_RANS_OK
+                                                                                        : FQXV_RANS_ERR_DEVICE;
+}
+
+int fqxv_rans_perm_join(const uint8_t *d_planes, uint64_t n, uint32_t *d_perm, void *stream)
+{
+    if (n && (!d_perm || !d_planes)) return FQXV_RANS_ERR_INVALID_ARG;
+    return fqxv_launch_perm_join(d_planes, n, d_perm, (hipStream_t)stream) == hipSuccess ? FQXV_RANS_OK
+                                                                                       : FQXV_RANS_ERR_DEVICE;
 }
 
 int fqxv_rans_decode_one_device(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,

@@ -205,6 +205,13 @@ int fqxv_rans_server_take(fqxv_rans_server *srv, fqxv_rans_done *out, uint32_t m
 /* Returns a completed request's output region to the pool. */
 int fqxv_rans_server_release(fqxv_rans_server *srv, uint64_t ticket);
 
+/* The reorder permutation's byte planes around the coder
+ * (crates/fqxv/src/container/reorder.rs:597-607 encode side,
+ * reorder.rs:1089-1100 decode side): plane k (bytes [k*n, (k+1)*n)) holds
+ * byte k of every u32 index.  Device pointers, asynchronous on `stream`. */
+int fqxv_rans_perm_split(const uint32_t *d_perm, uint64_t n, uint8_t *d_planes, void *stream);
+int fqxv_rans_perm_join(const uint8_t *d_planes, uint64_t n, uint32_t *d_perm, void *stream);
+
 /* Host helper: job order, largest `sizes[i]` first (ties by index). */
 void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);
 

@@ -218,3 +218,36 @@ def decode_many(arena: np.ndarray, offs, lens, out_caps, threads: int):
     lib().orc_decode_many(_ptr(arena), _ptr(offs), _ptr(lens), len(lens), _ptr(dst), _ptr(doff),
                           _ptr(caps), _ptr(out_len), _ptr(status), threads)
     return dst, doff, out_len, status
+
+
+# ---- reorder permutation byte planes (crates/fqxv/src/container/reorder.rs) ----
+class PermError(Exception):
+    """The container's `Error::Malformed` for a permutation payload."""
+
+
+def perm_split(perm) -> np.ndarray:
+    """reorder.rs:597-604: plane k (bytes [k*n, (k+1)*n)) = byte k of each index."""
+    p = np.ascontiguousarray(perm, dtype="<u4")
+    return p.view(np.uint8).reshape(-1, 4).T.copy().reshape(-1)
+
+
+def perm_join(planes, n: int) -> np.ndarray:
+    """reorder.rs:1097-1099: index i = LE bytes (planes[i], [n+i], [2n+i], [3n+i])."""
+    b = np.ascontiguousarray(planes, dtype=np.uint8)[: 4 * n].reshape(4, n).astype(np.uint32)
+    return b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24)
+
+
+def encode_permutation(perm) -> bytes:
+    """reorder.rs:597-607: planes, then the smaller rANS order (order-0 on a tie)."""
+    return encode_best(perm_split(perm).tobytes(), 0)
+
+
+def decode_permutation(stream: bytes, n: int) -> np.ndarray:
+    """reorder.rs:1089-1100: decode bounded by 4n, exact length, join."""
+    try:
+        pb = decode_bounded(stream, 4 * n)
+    except OracleError:
+        raise PermError("bad permutation")
+    if len(pb) != 4 * n:
+        raise PermError("permutation length mismatch")
+    return perm_join(np.frombuffer(pb, dtype=np.uint8), n)

@@ -353,3 +353,24 @@ def test_encode_best_follows_each_callers_tie_rule(oracle):
             assert got == (e1 if len(e1) < len(e0) else e0)
     assert oracle.encode_best(markov, 0)[0] == 1   # context helps Markov qualities
     assert oracle.encode_best(dna, 0)[0] == 0      # i.i.d. bases: order-1 only adds tables
+
+
+def test_permutation_planes_closed_form_and_errors(oracle):
+    """reorder.rs:597-607 / 1089-1100: plane k holds byte k of each index.
+    Hand example, split/join inverse, and the two container errors."""
+    p = np.array([0x04030201, 0x08070605], dtype=np.uint32)
+    assert oracle.perm_split(p).tolist() == [0x01, 0x05, 0x02, 0x06, 0x03, 0x07, 0x04, 0x08]
+    rng = np.random.default_rng(9)
+    for n in (0, 1, 7, 1000):
+        q = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        assert np.array_equal(oracle.perm_join(oracle.perm_split(q), n), q)
+    perm = rng.permutation(5000).astype(np.uint32)
+    s = oracle.encode_permutation(perm)
+    assert np.array_equal(oracle.decode_permutation(s, 5000), perm)
+    assert len(s) == min(len(oracle.encode(oracle.perm_split(perm).tobytes(), o)) for o in (0, 1))
+    with pytest.raises(oracle.PermError, match="^bad permutation$"):
+        oracle.decode_permutation(s[:20], 5000)
+    with pytest.raises(oracle.PermError, match="^bad permutation$"):
+        oracle.decode_permutation(s, 4999)          # declared 20000 > bound 19996
+    with pytest.raises(oracle.PermError, match="^permutation length mismatch$"):
+        oracle.decode_permutation(s, 5001)          # 20000 bytes, 20004 expected

@@ -165,6 +165,7 @@ def bench_c4(args, coder, dev, oracle):
     assert torch.equal(dec.dst[:n_total], arena[:n_total]), "device round trip differs"
     log(f"[c4] parity ok ({len(sample)} blocks vs oracle, all {nb} round-tripped)")
     eager_ms = batched_decode_eager_ms(coder, enc, comp_len, corpus.sizes)
+    crc = bench_crc(coder, oracle, enc, comp_len, args.workload) if args.crc else None
 
     out = dec.dst
     jobs_np = coder.jobs_array(enc.dst_off, comp_len, dec.dst_off, corpus.sizes)
@@ -225,6 +226,7 @@ def bench_c4(args, coder, dev, oracle):
     cpu_idx = np.arange(0, nb, args.cpu_every)
     host_c, host_off = slices_to_host(enc.dst, enc.dst_off[cpu_idx], comp_len[cpu_idx])
     res = {
+        "crc32c": crc,
         "encode_best": best,
         "corpus": corpus, "value": value, "t_step": t_step, "t_kern": t_kern, "n_total": n_total,
         "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "perblock": pb,
@@ -372,6 +374,42 @@ def bench_perm(coder, dev, oracle, n=1 << 26):
     return {"indices": n, "bytes_per_launch": 8 * n, **res}
 
 
+# ------------------------------------------------------------------------ CRC
+def bench_crc(coder, oracle, enc, comp_len, corpus_name):
+    """Container checksums (crates/fqxv/src/crc.rs) on the device: CRC-32C of
+    every compressed block (one range each) and of the whole compressed
+    arena as one range.  HBM-bound: algorithmic bytes = bytes read."""
+    import torch
+    stream = torch.cuda.current_stream()
+    offs = np.asarray(enc.dst_off, dtype=np.uint64)
+    lens = np.asarray(comp_len, dtype=np.uint64)
+    got = coder.crc32c_batch(enc.dst, offs, lens)
+    torch.cuda.synchronize()
+    idx = list(range(0, len(offs), max(1, len(offs) // 16)))
+    for j in idx:
+        o, n = int(offs[j]), int(lens[j])
+        ref = oracle.crc32c(enc.dst[o:o + n].cpu().numpy())
+        assert int(got[j]) & 0xFFFFFFFF == ref, j
+    res = {}
+    total = int(lens.sum())
+    span = int(offs[-1] + lens[-1])
+    for name, (o, l, nbytes) in {"per_block": (offs, lens, total),
+                                 "whole_arena": (np.zeros(1, np.uint64), np.array([span], np.uint64), span)}.items():
+        coder.crc32c_batch(enc.dst, o, l)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            coder.crc32c_batch(enc.dst, o, l)
+        t = (time.perf_counter() - t0) / reps
+        res[name] = {"ranges": len(o), "bytes": nbytes, "gbs": round(nbytes / t / 1e9, 1),
+                     "roofline_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 3)}
+    log(f"[crc] per-block {res['per_block']['gbs']} GB/s, whole arena {res['whole_arena']['gbs']} GB/s")
+    return {"corpus": f"compressed blocks of {corpus_name}", "checked_vs_oracle": len(idx),
+            "timing": "host wall clock per call (the call includes its host-side planning and returns "
+                      "when the checksums are written)", **res}
+
+
 # ------------------------------------------------------------------------ C5
 def bench_serve(args, coder, dev, state):
     import torch
@@ -494,6 +532,7 @@ def main():
     ap.add_argument("--perblock-every", type=int, default=20)
     ap.add_argument("--no-perblock", action="store_true")
     ap.add_argument("--encode-best", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--crc", action=argparse.BooleanOptionalAction, default=True)
     ap.add_argument("--cpu-every", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -579,6 +618,7 @@ def main():
         "tuner": {"grid": tuned.grid, "policy": tuned.policy, "table": tuned.table},
         "encode": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s", "form": "batched, one persistent launch"},
         "encode_best": c4["encode_best"],
+        "crc32c": c4["crc32c"],
         "configs": configs,
         "serving": serving,
     }

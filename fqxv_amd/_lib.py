@@ -37,7 +37,7 @@ EXPORTS = [
     "fqxv_rans_server_now", "fqxv_rans_server_submit",
     "fqxv_rans_server_poll", "fqxv_rans_server_take", "fqxv_rans_server_release",
     "fqxv_rans_encode_best_batch", "fqxv_rans_encode_best_bound",
-    "fqxv_rans_perm_split", "fqxv_rans_perm_join",
+    "fqxv_rans_perm_split", "fqxv_rans_perm_join", "fqxv_rans_crc32c_batch",
 ]
 
 
@@ -104,6 +104,7 @@ def load():
         L.fqxv_rans_ctx_set_grid.argtypes = [vp, ctypes.c_uint32]
         L.fqxv_rans_perm_split.argtypes = [vp, ctypes.c_uint64, vp, vp]
         L.fqxv_rans_perm_join.argtypes = [vp, ctypes.c_uint64, vp, vp]
+        L.fqxv_rans_crc32c_batch.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp, vp]
         u32 = ctypes.c_uint32
         L.fqxv_rans_server_create.argtypes = [ctypes.c_int, vp, vp, u32, vp, ctypes.c_uint64, u32, u32,
                                               ctypes.POINTER(vp)]

@@ -31,6 +31,10 @@ hipError_t fqxv_launch_encode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32
 hipError_t fqxv_launch_encode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
 hipError_t fqxv_encode_occupancy(int *);
 hipError_t fqxv_launch_perm_split(const u32 *, u64, u8 *, hipStream_t);
+size_t fqxv_crc_const_bytes();
+void fqxv_crc_const_fill(void *);
+hipError_t fqxv_launch_crc(const u8 *, const u64 *, const u64 *, const u32 *, const u32 *, u32, const u32 *, u32,
+                           const void *, u32 *, u32 *, u32, hipStream_t);
 hipError_t fqxv_launch_perm_join(const u8 *, u64, u32 *, hipStream_t);
 
 namespace {
@@ -63,6 +67,9 @@ struct fqxv_rans_ctx {
     u64 d_out_cap = 0;
     i32 *d_status = nullptr;
     u64 *d_out_len = nullptr;
+    void *crc_const = nullptr;              // slicing tables + zero-byte operators
+    u8 *crc_buf = nullptr;                  // per-call range/item arrays
+    u64 crc_cap = 0;
 };
 
 namespace {
@@ -214,6 +221,8 @@ void fqxv_rans_ctx_destroy(fqxv_rans_ctx *c)
         if (c->d_out) (void)hipFree(c->d_out);
         if (c->d_status) (void)hipFree(c->d_status);
         if (c->d_out_len) (void)hipFree(c->d_out_len);
+        if (c->crc_const) (void)hipFree(c->crc_const);
+        if (c->crc_buf) (void)hipFree(c->crc_buf);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -449,6 +458,57 @@ int fqxv_rans_perm_join(const uint8_t *d_planes, uint64_t n, uint32_t *d_perm, v
     if (n && (!d_perm || !d_planes)) return FQXV_RANS_ERR_INVALID_ARG;
     return fqxv_launch_perm_join(d_planes, n, d_perm, (hipStream_t)stream) == hipSuccess ? FQXV_RANS_OK
                                                                                        : FQXV_RANS_ERR_DEVICE;
+}
+
+int fqxv_rans_crc32c_batch(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
+                           uint32_t count, uint32_t *d_crc, void *stream)
+{
+    if (!c || (count && (!d_src || !off || !len || !d_crc))) return FQXV_RANS_ERR_INVALID_ARG;
+    if (count == 0) return FQXV_RANS_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    hipStream_t st = pick_stream(c, stream);
+    if (!c->crc_const) {
+        std::vector<u8> h(fqxv_crc_const_bytes());
+        fqxv_crc_const_fill(h.data());
+        if (hipMalloc(&c->crc_const, h.size()) != hipSuccess) {
+            c->crc_const = nullptr;
+            return FQXV_RANS_ERR_DEVICE;
+        }
+        if (hipMemcpy(c->crc_const, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return FQXV_RANS_ERR_DEVICE;
+    }
+    // Chunk items (range, chunk) in range order; first item of each range.
+    std::vector<u32> item_range, item_chunk, first(count);
+    for (u32 r = 0; r < count; r++) {
+        if (len[r] >> 47) return FQXV_RANS_ERR_INVALID_ARG;
+        first[r] = (u32)item_range.size();
+        const u64 nch = (len[r] + 65535) >> 16;
+        if (item_range.size() + nch > 0xFFFFFFFFull) return FQXV_RANS_ERR_INVALID_ARG;
+        for (u64 k = 0; k < nch; k++) {
+            item_range.push_back(r);
+            item_chunk.push_back((u32)k);
+        }
+    }
+    const u64 ni = item_range.size();
+    // Device layout: off[count] len[count] (u64), first[count] item_range[ni]
+    // item_chunk[ni] raw[ni] (u32).
+    const u64 need = 16ull * count + 4ull * count + 12ull * ni + 64;
+    int rc = ensure_buf(&c->crc_buf, &c->crc_cap, need);
+    if (rc) return rc;
+    u64 *d_off = (u64 *)c->crc_buf, *d_len = d_off + count;
+    u32 *d_first = (u32 *)(d_len + count), *d_ir = d_first + count, *d_ic = d_ir + ni, *d_raw = d_ic + ni;
+    if (hipMemcpyAsync(d_off, off, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_len, len, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_first, first.data(), 4ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (ni && hipMemcpyAsync(d_ir, item_range.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (ni && hipMemcpyAsync(d_ic, item_chunk.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess))
+        return FQXV_RANS_ERR_DEVICE;
+    if (fqxv_launch_crc(d_src, d_off, d_len, d_ir, d_ic, (u32)ni, d_first, count, c->crc_const, d_raw, d_crc,
+                        std::max<u32>(c->dec_max, 256), st) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    // The staging vectors above are host locals: finish before returning.
+    return hipStreamSynchronize(st) == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
 }
 
 int fqxv_rans_decode_one_device(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,

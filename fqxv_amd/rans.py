@@ -294,6 +294,18 @@ class Coder:
         _raise(self._L.fqxv_rans_max_waves(self._h, ctypes.byref(w)))
         return int(w.value)
 
+    def crc32c_batch(self, src, offs, lens, stream=None):
+        """CRC-32C (crates/fqxv/src/crc.rs) of byte ranges of a device arena;
+        returns a device uint32 (int32-typed) tensor."""
+        import torch
+        offs = np.ascontiguousarray(offs, dtype=np.uint64)
+        lens = np.ascontiguousarray(lens, dtype=np.uint64)
+        out = torch.zeros(max(len(offs), 1), dtype=torch.int32, device=src.device)
+        _raise(self._L.fqxv_rans_crc32c_batch(self._h, ctypes.c_void_p(src.data_ptr()),
+                                              ctypes.c_void_p(offs.ctypes.data), ctypes.c_void_p(lens.ctypes.data),
+                                              len(offs), ctypes.c_void_p(out.data_ptr()), self._stream(stream)))
+        return out[: len(offs)]
+
     def set_grid(self, waves: int) -> None:
         """Persistent-grid size of later batched calls (0 = occupancy max)."""
         _raise(self._L.fqxv_rans_ctx_set_grid(self._h, int(waves)))

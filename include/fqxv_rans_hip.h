@@ -212,6 +212,15 @@ int fqxv_rans_server_release(fqxv_rans_server *srv, uint64_t ticket);
 int fqxv_rans_perm_split(const uint32_t *d_perm, uint64_t n, uint8_t *d_planes, void *stream);
 int fqxv_rans_perm_join(const uint8_t *d_planes, uint64_t n, uint32_t *d_perm, void *stream);
 
+/* CRC-32C (crates/fqxv/src/crc.rs: reflected Castagnoli 0x82F63B78, init and
+ * xor-out 0xFFFFFFFF) of `count` byte ranges [off[i], off[i] + len[i]) of the
+ * device arena `d_src` -- the container's frame, per-stream, whole-file and
+ * footer checksums (crates/fqxv/src/container/mod.rs:53-97).  `off` and
+ * `len` are host arrays (len < 2^47); d_crc[i] receives the checksum.  Work
+ * runs on `stream`; the call returns when the checksums are written. */
+int fqxv_rans_crc32c_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
+                           uint32_t count, uint32_t *d_crc, void *stream);
+
 /* Host helper: job order, largest `sizes[i]` first (ties by index). */
 void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);
 

@@ -80,6 +80,12 @@ def lib():
             fn.restype = ctypes.c_int
         L.orc_encode_many.argtypes = [p8, p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
         L.orc_decode_many.argtypes = [p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
+        L.orc_crc32c.restype = ctypes.c_uint32
+        L.orc_crc32c.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        L.orc_crc32c_combine.restype = ctypes.c_uint32
+        L.orc_crc32c_combine.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+        L.orc_crc32c_many.restype = None
+        L.orc_crc32c_many.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -251,3 +257,21 @@ def decode_permutation(stream: bytes, n: int) -> np.ndarray:
     if len(pb) != 4 * n:
         raise PermError("permutation length mismatch")
     return perm_join(np.frombuffer(pb, dtype=np.uint8), n)
+
+
+# ---- CRC-32C (crates/fqxv/src/crc.rs) ----------------------------------------
+def crc32c(data) -> int:
+    b = _as_u8(data)
+    return int(lib().orc_crc32c(_ptr(b) if b.size else None, b.size))
+
+
+def crc32c_combine(crc_a: int, crc_b: int, len_b: int) -> int:
+    return int(lib().orc_crc32c_combine(crc_a, crc_b, len_b))
+
+
+def crc32c_many(arena: np.ndarray, offs, lens) -> np.ndarray:
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint64)
+    out = np.zeros(len(offs), dtype=np.uint32)
+    lib().orc_crc32c_many(_ptr(arena), _ptr(offs), _ptr(lens), len(offs), _ptr(out))
+    return out

@@ -534,3 +534,70 @@ int orc_decode_many(const uint8_t *src, const uint64_t *src_off, const uint64_t 
                   .status = status, .count = count};
     return run_many(&j, threads);
 }
+
+/* ---- CRC-32C (crates/fqxv/src/crc.rs) ------------------------------------
+ * Reflected Castagnoli polynomial 0x82F63B78, init and xor-out 0xFFFFFFFF,
+ * byte-wise table (crc.rs:15-36, update crc.rs:51-58), and the zlib-style
+ * combine over GF(2) operator matrices (crc.rs:84-160). */
+static uint32_t crc_table[256];
+static int crc_table_ready;
+
+static void crc_init_table(void)
+{
+    for (uint32_t i = 0; i < 256; i++) {
+        uint32_t c = i;
+        for (int b = 0; b < 8; b++) c = (c & 1) ? (c >> 1) ^ 0x82F63B78u : c >> 1;
+        crc_table[i] = c;
+    }
+    crc_table_ready = 1;
+}
+
+uint32_t orc_crc32c_update(uint32_t state, const uint8_t *p, size_t n)
+{
+    if (!crc_table_ready) crc_init_table();
+    for (size_t i = 0; i < n; i++) state = crc_table[(state ^ p[i]) & 0xff] ^ (state >> 8);
+    return state;
+}
+
+uint32_t orc_crc32c(const uint8_t *p, size_t n) { return orc_crc32c_update(0xFFFFFFFFu, p, n) ^ 0xFFFFFFFFu; }
+
+static uint32_t gf2_times(const uint32_t mat[32], uint32_t vec)
+{
+    uint32_t sum = 0;
+    for (int i = 0; vec; i++, vec >>= 1)
+        if (vec & 1) sum ^= mat[i];
+    return sum;
+}
+
+static void gf2_square(uint32_t sq[32], const uint32_t mat[32])
+{
+    for (int n = 0; n < 32; n++) sq[n] = gf2_times(mat, mat[n]);
+}
+
+uint32_t orc_crc32c_combine(uint32_t crc_a, uint32_t crc_b, uint64_t len_b)
+{
+    if (len_b == 0) return crc_a;
+    uint32_t odd[32], even[32];
+    odd[0] = 0x82F63B78u;
+    for (int i = 1; i < 32; i++) odd[i] = 1u << (i - 1);
+    gf2_square(even, odd);
+    gf2_square(odd, even);
+    uint32_t crc = crc_a;
+    for (;;) {
+        gf2_square(even, odd);
+        if (len_b & 1) crc = gf2_times(even, crc);
+        len_b >>= 1;
+        if (!len_b) break;
+        gf2_square(odd, even);
+        if (len_b & 1) crc = gf2_times(odd, crc);
+        len_b >>= 1;
+        if (!len_b) break;
+    }
+    return crc ^ crc_b;
+}
+
+void orc_crc32c_many(const uint8_t *src, const uint64_t *off, const uint64_t *len, uint32_t count,
+                     uint32_t *crc_out)
+{
+    for (uint32_t i = 0; i < count; i++) crc_out[i] = orc_crc32c(src + off[i], (size_t)len[i]);
+}

@@ -82,4 +82,11 @@ int orc_decode_many(const uint8_t *src, const uint64_t *src_off,
 #ifdef __cplusplus
 }
 #endif
+/* CRC-32C of crates/fqxv/src/crc.rs (container integrity checksums). */
+uint32_t orc_crc32c_update(uint32_t state, const uint8_t *p, size_t n);
+uint32_t orc_crc32c(const uint8_t *p, size_t n);
+uint32_t orc_crc32c_combine(uint32_t crc_a, uint32_t crc_b, uint64_t len_b);
+void orc_crc32c_many(const uint8_t *src, const uint64_t *off, const uint64_t *len, uint32_t count,
+                     uint32_t *crc_out);
+
 #endif

@@ -374,3 +374,25 @@ def test_permutation_planes_closed_form_and_errors(oracle):
         oracle.decode_permutation(s, 4999)          # declared 20000 > bound 19996
     with pytest.raises(oracle.PermError, match="^permutation length mismatch$"):
         oracle.decode_permutation(s, 5001)          # 20000 bytes, 20004 expected
+
+
+def test_crc32c_matches_reference_written_checksums(oracle):
+    """crates/fqxv/src/crc.rs restated: the standard check value, and the 14
+    CRC-32C values stored in two reference-written archives (header, frame,
+    per-stream, whole-file and footer checksums; tests/golden/crc_vectors.json
+    from make_crc_vectors.py)."""
+    import json
+    import os
+    assert oracle.crc32c(b"123456789") == 0xE3069283
+    assert oracle.crc32c(b"") == 0
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    vecs = json.load(open(os.path.join(g, "crc_vectors.json")))
+    assert len(vecs) == 14
+    for e in vecs:
+        b = open(os.path.join(g, f"crc_{e['archive']}.bin"), "rb").read()
+        assert oracle.crc32c(b[e["off"]:e["off"] + e["len"]]) == e["crc"], e
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 256, 5000, dtype=np.uint8).tobytes()
+    for cut in (0, 1, 999, 5000):
+        a, b = x[:cut], x[cut:]
+        assert oracle.crc32c_combine(oracle.crc32c(a), oracle.crc32c(b), len(b)) == oracle.crc32c(x)

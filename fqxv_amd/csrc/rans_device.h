@@ -46,16 +46,29 @@ __device__ __forceinline__ u32 rank_below(u64 m)
 }
 
 __device__ __forceinline__ u32 uni(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ u64 uni64(u64 v) { return (u64)uni((u32)v) | ((u64)uni((u32)(v >> 32)) << 32); }
+
+// A job record read by the whole wave, made wave-uniform: every value derived
+// from it (lengths, offsets, loop bounds) then lives in scalar registers and
+// loops over it stay scalar loops.
+__device__ __forceinline__ fqxv_rans_job uni_job(const fqxv_rans_job &j)
+{
+    fqxv_rans_job u;
+    u.src_off = uni64(j.src_off);
+    u.src_len = uni64(j.src_len);
+    u.dst_off = uni64(j.dst_off);
+    u.dst_cap = uni64(j.dst_cap);
+    u.max_len = uni64(j.max_len);
+    u.order = (i32)uni((u32)j.order);
+    u.flags = uni(j.flags);
+    return u;
+}
 
 // v_writelane_b32 (the LLVM intrinsic; clang exposes no builtin for it).
 extern "C" __device__ int fqxv_llvm_writelane(int, int, int) __asm("llvm.amdgcn.writelane.i32");
 __device__ __forceinline__ u32 writelane(u32 val, u32 lane, u32 old)
 {
     return (u32)fqxv_llvm_writelane((int)uni(val), (int)lane, (int)old);
-}
-__device__ __forceinline__ u64 uni64(u64 v)
-{
-    return ((u64)uni((u32)(v >> 32)) << 32) | (u64)uni((u32)v);
 }
 
 __device__ __forceinline__ u32 wave_sum(u32 v)

@@ -41,7 +41,7 @@ struct EncShared {
     u8 colmap[256];         // symbol -> dense column (order-1 dense mode)
     u8 rowmap[256];         // context -> dense row
     u8 ctx_of[256];         // k-th present context (ascending)
-    u32 stage[2][CH_DW + 3];
+    u32 stage[2][CH_DW + 12];   // + slack: lanes past the chunk read inside it
 };
 
 // Bytes of p[0..3] that fall in [begin, end), others zero.
@@ -70,7 +70,7 @@ __device__ __forceinline__ u32 pick4(const u32 v[4], u32 i)
 }
 
 // model.rs:213-264 over 256 counts, lane l owning symbols 4l..4l+3.
-__device__ void normalize_wave(const u32 c[4], u32 f[4])
+__device__ __forceinline__ void normalize_wave(const u32 c[4], u32 f[4])
 {
     const u32 l = lane_id();
     const u64 total = wave_sum64((u64)c[0] + c[1] + c[2] + c[3]);
@@ -167,18 +167,86 @@ struct TabRef {
     u32 mode;   // 0: order-0 (index = sym); 1: dense order-1; 2: global order-1
 };
 
-__device__ __forceinline__ u32 tab_index(const TabRef &t, u32 ctx, u32 sym)
+
+template <u32 MODE>
+__device__ __forceinline__ u32 tab_index_t(const TabRef &t, u32 ctx, u32 sym)
 {
-    if (t.mode == 0) return sym;
-    if (t.mode == 1) return (u32)t.rowmap[ctx] * DENSE_COLS + t.colmap[sym];
+    if (MODE == 0) return sym;
+    if (MODE == 1) return (u32)t.rowmap[ctx] * DENSE_COLS + t.colmap[sym];
     return ctx * 256 + sym;
 }
 
-// The reverse walk over all n positions; returns the number of words.
-__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u32 &x)
+// One round's table entry for lane l at position i (staged chunk sb, chunk
+// start p0).  Inactive lanes get f = 4096 (never renormalise, x unchanged).
+// Every read is unconditional: lanes past the chunk read the stage slack,
+// and an inactive lane's table index is forced to 0.  Modes 0 and 1 read
+// the LDS tables directly, mode 2 the workspace tables.
+template <u32 MODE>
+__device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &tr, const u8 *sb, u64 p0, u64 r,
+                                            u64 n, u32 &fcm, u32 &mg)
 {
     const u32 l = lane_id();
-    x = RANS_L;
+    const u64 i = r * NSTATES + l;
+    const bool act = l < NSTATES && i < n;
+    // Lanes 32..63 address as lanes 0..31: every read stays within the
+    // chunk's stage (at most 31 positions past its end, inside the slack).
+    const u32 t = (u32)(r * NSTATES + (l & (NSTATES - 1)) - p0);
+    const u32 sym = sb[4 + t];
+    const u32 ctx = i > 0 ? sb[3 + t] : 0u;
+    u32 e;
+    if (MODE == 0)
+        e = sym;
+    else if (MODE == 1)
+        e = (u32)sh.rowmap[ctx] * DENSE_COLS + sh.colmap[sym];
+    else
+        e = ctx * 256 + sym;
+    e = act ? e : 0u;
+    const u32 fv = MODE == 2 ? tr.fc[e] : sh.fc[e];
+    const u32 mv = MODE == 2 ? tr.mg[e] : sh.mg[e];
+    fcm = act ? fv : (TOTFREQ << 12);
+    mg = act ? mv : 1u;
+}
+
+// One encode step of every lane (scalar.rs:120-127 per lane; lanes emit in
+// descending position order within the round).
+__device__ __forceinline__ void round_step(u32 &x, u64 &cnt, u32 fcm, u32 mg, u8 *wend, bool even)
+{
+    const u32 f = fcm >> 12, cum = fcm & 0xfff;
+    // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73); inactive
+    // lanes carry f = 4096, which no 32-bit state reaches.
+    const bool need = (x >> 20) >= f;
+    const u64 m = ballot(need);
+    if (need) {
+        // Emission index in the reference's renorm vector: higher lanes of
+        // the round come first (descending positions).
+        const u64 idx = cnt + ((u32)__popcll(m) - 1 - rank_below(m));
+        u8 *w = wend - 2 * (idx + 1);
+        if (even) {
+            *(u16 *)w = (u16)x;
+        } else {
+            w[0] = (u8)x;
+            w[1] = (u8)(x >> 8);
+        }
+    }
+    const u32 v = need ? (x >> 16) : x;
+    u32 q = __umulhi(v, mg);
+    u32 rr = v - q * f;
+    if (rr >= f) {
+        q++;
+        rr -= f;
+    }
+    x = (q << SCALE_BITS) + rr + cum;
+    cnt += (u32)__popcll(m);
+}
+
+// The reverse walk over all n positions; returns the number of words.  The
+// table entries of four rounds are read before their four state updates, so
+// the LDS latency is paid once per four rounds on the state chain.
+template <u32 MODE>
+__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u32 &x_out)
+{
+    const bool even = ((uintptr_t)wend & 1) == 0;
+    u32 x = RANS_L; // a register for the whole walk, written back at the end
     u64 cnt = 0;
     const u64 nch = (n + CH - 1) / CH;
     u32 pf[9];
@@ -191,37 +259,20 @@ __device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *we
         const u8 *sb = (const u8 *)sh.stage[ch & 1];
         const u64 pend = p0 + CH < n ? p0 + CH : n;
         const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
-        for (u64 r = r_hi + 1; r-- > r_lo;) {
-            const u64 i = r * NSTATES + l;
-            const bool act = l < NSTATES && i < n;
-            const u32 t = (u32)(i - p0);
-            const u32 sym = act ? sb[4 + t] : 0u;
-            const u32 ctx = (act && i > 0) ? sb[3 + t] : 0u;
-            const u32 e = tab_index(tr, ctx, sym);
-            const u32 fcm = act ? tr.fc[e] : (TOTFREQ << 12);
-            const u32 mg = act ? tr.mg[e] : 1u;
-            const u32 f = fcm >> 12, cum = fcm & 0xfff;
-            // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73).
-            const bool need = act && (x >> 20) >= f;
-            const u64 m = ballot(need);
-            const u32 k = (u32)__popcll(m);
-            if (need) {
-                // Emission index in the reference's renorm vector: higher
-                // lanes of the round come first (descending positions).
-                const u64 idx = cnt + (k - 1 - rank_below(m));
-                u8 *w = wend - 2 * (idx + 1);
-                w[0] = (u8)x;
-                w[1] = (u8)(x >> 8);
-            }
-            const u32 v = need ? (x >> 16) : x;
-            u32 q = __umulhi(v, mg);
-            u32 rr = v - q * f;
-            if (rr >= f) {
-                q++;
-                rr -= f;
-            }
-            if (act) x = (q << SCALE_BITS) + rr + cum;
-            cnt += k;
+        u64 r = r_hi + 1; // rounds r-1 down to r_lo remain
+        while (r >= r_lo + 4) {
+            u32 fcm[4], mg[4];
+#pragma unroll
+            for (u32 k = 0; k < 4; k++) round_entry<MODE>(sh, tr, sb, p0, r - 1 - k, n, fcm[k], mg[k]);
+#pragma unroll
+            for (u32 k = 0; k < 4; k++) round_step(x, cnt, fcm[k], mg[k], wend, even);
+            r -= 4;
+        }
+        while (r > r_lo) {
+            u32 fcm, mg;
+            round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);
+            round_step(x, cnt, fcm, mg, wend, even);
+            r--;
         }
         if (ch > 0) {
             __syncthreads();
@@ -229,6 +280,7 @@ __device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *we
             __syncthreads();
         }
     }
+    x_out = x;
     return cnt;
 }
 
@@ -280,9 +332,10 @@ __device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts)
     __syncthreads();
 }
 
-__device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, u64 &out_len,
+__device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb_in, u64 &out_len,
                           u8 *ws)
 {
+    const fqxv_rans_job jb = uni_job(jb_in);
     const u32 l = lane_id();
     const u8 *in = src + jb.src_off;
     u8 *o = dst + jb.dst_off;
@@ -316,7 +369,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         __syncthreads();
         TabRef tr{sh.fc, sh.mg, sh.rowmap, sh.colmap, 0};
         u32 x;
-        const u64 W = walk(sh, in, n, tr, wend, x);
+        const u64 W = walk<0>(sh, in, n, tr, wend, x);
         finish(o, O0_HDR, wend, W, x);
         out_len = O0_HDR + 2 * W;
         return 0;
@@ -414,7 +467,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u}
                       : TabRef{gfc, gmg, sh.rowmap, sh.colmap, 2u};
     u32 x;
-    const u64 W = walk(sh, in, n, tr, wend, x);
+    const u64 W = dense ? walk<1>(sh, in, n, tr, wend, x) : walk<2>(sh, in, n, tr, wend, x);
     finish(o, hdr, wend, W, x);
     out_len = hdr + 2 * W;
     return 0;
@@ -432,9 +485,10 @@ __device__ __forceinline__ u64 encode_bound_dev(u64 n, int order)
 // `jb.order` names the order kept on a tie.  Order-0 is written at the front
 // of the region, order-1 behind it; an order-1 winner slides down to the
 // front.  The stream's first byte records the choice.
-__device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, u64 &out_len,
+__device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb_in, u64 &out_len,
                                u8 *ws)
 {
+    const fqxv_rans_job jb = uni_job(jb_in);
     const int tie = jb.order;
     if (tie != 0 && tie != 1) return FQXV_RANS_ERR_INVALID_ARG;
     if (jb.src_len >= (1ull << 32)) return FQXV_RANS_ERR_INVALID_ARG;
@@ -475,8 +529,8 @@ __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, 
         if (lane_id() == 0) w = atomicAdd(counter, 1u);
         w = uni(__shfl(w, 0));
         if (w >= count) break;
-        const u32 j = order ? order[w] : w;
-        const fqxv_rans_job jb = jobs[j];
+        const u32 j = order ? uni(order[w]) : w;
+        const fqxv_rans_job jb = uni_job(jobs[j]);
         u64 ol = 0;
         const int err = best ? encode_best_job(sh, src, dst, jb, ol, myws) : encode_job(sh, src, dst, jb, ol, myws);
         if (lane_id() == 0) {

@@ -33,6 +33,9 @@ constexpr u32 DENSE_ROWS = 65;           // 64 dense symbols + context 0 when ab
 constexpr u32 TAB_ENTRIES = DENSE_ROWS * DENSE_COLS;
 constexpr u32 CH = 2048;                 // walk staging chunk (positions)
 constexpr u32 CH_DW = CH / 4 + 1;        // + 4 leading bytes (context of the first)
+#ifndef WALK_BATCH
+#define WALK_BATCH 8                      // rounds whose table entries are read together
+#endif
 
 struct EncShared {
     u32 fc[TAB_ENTRIES];    // counts, then f << 12 | cum
@@ -208,21 +211,23 @@ __device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &t
 }
 
 // One encode step of every lane (scalar.rs:120-127 per lane; lanes emit in
-// descending position order within the round).
-__device__ __forceinline__ void round_step(u32 &x, u64 &cnt, u32 fcm, u32 mg, u8 *wend, bool even)
+// descending position order within the round).  The round's k words go to
+// [wend - 2(cnt + k), wend - 2 cnt): a wave-uniform base plus 2 * rank.
+typedef __attribute__((address_space(1))) u8 gu8;
+typedef __attribute__((address_space(1))) u16 gu16;
+__device__ __forceinline__ void round_step(u32 &x, u64 &cnt, u32 fcm, u32 mg, gu8 *wend, bool even)
 {
     const u32 f = fcm >> 12, cum = fcm & 0xfff;
     // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73); inactive
     // lanes carry f = 4096, which no 32-bit state reaches.
     const bool need = (x >> 20) >= f;
     const u64 m = ballot(need);
+    const u32 k = (u32)__popcll(m);
     if (need) {
-        // Emission index in the reference's renorm vector: higher lanes of
-        // the round come first (descending positions).
-        const u64 idx = cnt + ((u32)__popcll(m) - 1 - rank_below(m));
-        u8 *w = wend - 2 * (idx + 1);
+        // Emission index cnt + (k - 1 - rank): higher lanes first.
+        gu8 *w = wend - 2 * (cnt + k) + 2 * rank_below(m);
         if (even) {
-            *(u16 *)w = (u16)x;
+            *(gu16 *)w = (u16)x;
         } else {
             w[0] = (u8)x;
             w[1] = (u8)(x >> 8);
@@ -236,16 +241,17 @@ __device__ __forceinline__ void round_step(u32 &x, u64 &cnt, u32 fcm, u32 mg, u8
         rr -= f;
     }
     x = (q << SCALE_BITS) + rr + cum;
-    cnt += (u32)__popcll(m);
+    cnt += k;
 }
 
 // The reverse walk over all n positions; returns the number of words.  The
-// table entries of four rounds are read before their four state updates, so
-// the LDS latency is paid once per four rounds on the state chain.
+// table entries of WALK_BATCH rounds are read before their state updates,
+// so the LDS latency is paid once per batch on the state chain.
 template <u32 MODE>
-__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u32 &x_out)
+__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend_generic, u32 &x_out)
 {
-    const bool even = ((uintptr_t)wend & 1) == 0;
+    gu8 *wend = (gu8 *)wend_generic;
+    const bool even = ((uintptr_t)wend_generic & 1) == 0;
     u32 x = RANS_L; // a register for the whole walk, written back at the end
     u64 cnt = 0;
     const u64 nch = (n + CH - 1) / CH;
@@ -260,13 +266,13 @@ __device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *we
         const u64 pend = p0 + CH < n ? p0 + CH : n;
         const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
         u64 r = r_hi + 1; // rounds r-1 down to r_lo remain
-        while (r >= r_lo + 4) {
-            u32 fcm[4], mg[4];
+        while (r >= r_lo + WALK_BATCH) {
+            u32 fcm[WALK_BATCH], mg[WALK_BATCH];
 #pragma unroll
-            for (u32 k = 0; k < 4; k++) round_entry<MODE>(sh, tr, sb, p0, r - 1 - k, n, fcm[k], mg[k]);
+            for (u32 k = 0; k < WALK_BATCH; k++) round_entry<MODE>(sh, tr, sb, p0, r - 1 - k, n, fcm[k], mg[k]);
 #pragma unroll
-            for (u32 k = 0; k < 4; k++) round_step(x, cnt, fcm[k], mg[k], wend, even);
-            r -= 4;
+            for (u32 k = 0; k < WALK_BATCH; k++) round_step(x, cnt, fcm[k], mg[k], wend, even);
+            r -= WALK_BATCH;
         }
         while (r > r_lo) {
             u32 fcm, mg;

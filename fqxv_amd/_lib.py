@@ -24,6 +24,7 @@ ERR_BOUND = 7
 ERR_DST_TOO_SMALL = 8
 ERR_INVALID_ARG = 9
 ERR_DEVICE = 10
+NAMES_ERR_FIRST = 32
 
 # Every symbol include/fqxv_rans_hip.h declares.
 EXPORTS = [
@@ -38,6 +39,8 @@ EXPORTS = [
     "fqxv_rans_server_poll", "fqxv_rans_server_take", "fqxv_rans_server_release",
     "fqxv_rans_encode_best_batch", "fqxv_rans_encode_best_bound",
     "fqxv_rans_perm_split", "fqxv_rans_perm_join", "fqxv_rans_crc32c_batch",
+    "fqxv_names_decode_batch", "fqxv_names_status", "fqxv_names_records",
+    "fqxv_names_batch_destroy",
 ]
 
 
@@ -115,6 +118,13 @@ def load():
         L.fqxv_rans_server_poll.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]
         L.fqxv_rans_server_take.argtypes = [vp, ctypes.POINTER(Done), u32, ctypes.POINTER(u32)]
         L.fqxv_rans_server_release.argtypes = [vp, ctypes.c_uint64]
+        L.fqxv_names_decode_batch.argtypes = [vp, vp, vp, vp, u32, ctypes.POINTER(vp)]
+        L.fqxv_names_status.argtypes = [vp, u32]
+        L.fqxv_names_status.restype = ctypes.c_int32
+        L.fqxv_names_records.argtypes = [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_uint64)]
+        L.fqxv_names_batch_destroy.argtypes = [vp]
+        L.fqxv_names_batch_destroy.restype = None
         _lib = L
         return L
 

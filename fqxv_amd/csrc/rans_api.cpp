@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/fqxv_rans_hip.h"
+#include "rans_internal.h"
 
 typedef uint8_t u8;
 typedef uint32_t u32;
@@ -70,6 +71,8 @@ struct fqxv_rans_ctx {
     void *crc_const = nullptr;              // slicing tables + zero-byte operators
     u8 *crc_buf = nullptr;                  // per-call range/item arrays
     u64 crc_cap = 0;
+    u8 *job_buf = nullptr;                  // host-staged batches: jobs, order, status, out_len
+    u64 job_cap = 0;
 };
 
 namespace {
@@ -155,6 +158,21 @@ const char *fqxv_rans_strerror(int code)
     case FQXV_RANS_ERR_DST_TOO_SMALL: return "destination buffer too small";
     case FQXV_RANS_ERR_INVALID_ARG: return "invalid argument";
     case FQXV_RANS_ERR_DEVICE: return "HIP device error";
+    case FQXV_NAMES_ERR_TRUNCATED: return "truncated";
+    case FQXV_NAMES_ERR_VARINT: return "varint too long";
+    case FQXV_NAMES_ERR_VERSION: return "unsupported version";
+    case FQXV_NAMES_ERR_RECORD_COUNT: return "record count exceeds token stream";
+    case FQXV_NAMES_ERR_OP_SYM_UNDERRUN: return "op sym underrun";
+    case FQXV_NAMES_ERR_OP_RUN_TOO_LONG: return "op run too long";
+    case FQXV_NAMES_ERR_OP_BOUND: return "op columns exceed the stream bound";
+    case FQXV_NAMES_ERR_PLANE_LEN: return "delta plane length mismatch";
+    case FQXV_NAMES_ERR_MATCH: return "MATCH without prior token";
+    case FQXV_NAMES_ERR_STR_UNDERRUN: return "string data underrun";
+    case FQXV_NAMES_ERR_PAD_UNDERRUN: return "pad underrun";
+    case FQXV_NAMES_ERR_DELTA_UNDERRUN: return "delta underrun";
+    case FQXV_NAMES_ERR_DELTA_PRIOR: return "DELTA without numeric prior";
+    case FQXV_NAMES_ERR_UNKNOWN_OP: return "unknown op";
+    case FQXV_NAMES_ERR_ALLOC: return "op run too large to allocate";
     default: return "unknown error";
     }
 }
@@ -223,6 +241,7 @@ void fqxv_rans_ctx_destroy(fqxv_rans_ctx *c)
         if (c->d_out_len) (void)hipFree(c->d_out_len);
         if (c->crc_const) (void)hipFree(c->crc_const);
         if (c->crc_buf) (void)hipFree(c->crc_buf);
+        if (c->job_buf) (void)hipFree(c->job_buf);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -340,6 +359,52 @@ static int batch_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, 
                              : fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, ctr, d_status,
                                                         d_out_len, c->ws, grid, st);
     return e == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
+}
+
+// Host-staged batch: the source span and the jobs go up, one persistent
+// decode launch runs (largest output first), status, lengths and the whole
+// destination arena come back.  Serves the names decoder.
+int fqxv_decode_jobs_host(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_len, const fqxv_rans_job *h_jobs,
+                          uint32_t count, uint64_t dst_total, uint8_t *h_dst, int32_t *h_status,
+                          uint64_t *h_out_len)
+{
+    if (!c || (src_len && !h_src) || (count && (!h_jobs || !h_status || !h_out_len)) || (dst_total && !h_dst))
+        return FQXV_RANS_ERR_INVALID_ARG;
+    if (count == 0) return FQXV_RANS_OK;
+    std::vector<u64> cost(count);
+    for (u32 i = 0; i < count; i++) {
+        const fqxv_rans_job &j = h_jobs[i];
+        if (j.src_off + j.src_len > src_len || j.dst_off + j.dst_cap > dst_total) return FQXV_RANS_ERR_INVALID_ARG;
+        cost[i] = j.dst_cap + j.src_len;
+    }
+    std::vector<u32> order(count);
+    fqxv_rans_lpt_order(cost.data(), count, order.data());
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    const u32 grid = std::min<u32>(count, c->dec_waves);
+    const u64 jb = sizeof(fqxv_rans_job) * (u64)count, ob = 4ull * count, sb = 4ull * count, lb = 8ull * count;
+    int rc;
+    if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, src_len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dst_total)) ||
+        (rc = ensure_buf(&c->job_buf, &c->job_cap, lb + jb + ob + sb)) || (rc = ensure_ws(c, grid)))
+        return rc;
+    u64 *d_len = (u64 *)c->job_buf;
+    fqxv_rans_job *d_jobs = (fqxv_rans_job *)(d_len + count);
+    u32 *d_order = (u32 *)(d_jobs + count);
+    i32 *d_st = (i32 *)(d_order + count);
+    hipStream_t st = c->stream;
+    if ((src_len && hipMemcpyAsync(c->d_in, h_src, src_len, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipMemcpyAsync(d_jobs, h_jobs, jb, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_order, order.data(), ob, hipMemcpyHostToDevice, st) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    if (fqxv_launch_decode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len, c->ws,
+                                 grid, st) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    if (hipMemcpyAsync(h_status, d_st, sb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(h_out_len, d_len, lb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (dst_total && hipMemcpyAsync(h_dst, c->d_out, dst_total, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    return FQXV_RANS_OK;
 }
 
 int fqxv_rans_decode_batch(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,

@@ -1,0 +1,427 @@
+// Batched decode of read-name streams (crates/fqxv-tokenizer/src/lib.rs,
+// format version 1).
+//
+// A names stream is a version byte, a varint record count, eight rANS
+// sub-streams framed as [varint len][bytes] (token counts, string lengths,
+// string bytes, numeric literals, paddings, and the op run-length triple),
+// then per token column (64) a varint plane width and that many framed
+// delta byte-plane sub-streams (lib.rs:98-169, 515-537).  The framing is
+// varints and lengths only -- none of it depends on decoded data -- so:
+//   1. the host walks the framing of every stream and records its steps in
+//      the reference's read order (sub-stream, column width, or the framing
+//      error that ends the walk);
+//   2. every rANS sub-stream of every stream is decoded in ONE batched,
+//      persistent device launch;
+//   3. the host replays each stream's steps with the decoded bytes -- the
+//      checks of lib.rs:543-588 in order, op run-length expansion
+//      (lib.rs:202-245), byte-plane assembly (lib.rs:137-168) -- and
+//      rebuilds the names (lib.rs:590-650), threads across streams.
+// Names come out in the layout the container's names digest covers
+// (crates/fqxv/src/container/block.rs:164-176): [u64 n] then per name
+// [u32 len][bytes], little-endian.
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/fqxv_rans_hip.h"
+#include "rans_internal.h"
+
+typedef uint8_t u8;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int32_t i32;
+typedef int64_t i64;
+
+namespace {
+
+constexpr u8 FORMAT_VERSION = 1;                       // lib.rs:55
+constexpr u8 OP_MATCH = 0, OP_STR = 1, OP_NUM = 2, OP_DELTA = 3; // lib.rs:58-61
+constexpr u32 NCOL = 64;                               // MAX_COL + 1 (lib.rs:66)
+constexpr u64 MAX_STREAM_EXPANSION = 1ull << 16;       // lib.rs:75
+constexpr u64 MIN_STREAM_BOUND = 1ull << 20;           // lib.rs:76
+constexpr u32 NFLAT = 8;                               // sub-streams before the planes
+
+// A cursor with the tokenizer's error codes (fqxv-bytes Reader through
+// lib.rs:700-712): a varint that runs off the end is "varint too long".
+struct Cur {
+    const u8 *p;
+    u64 n, pos = 0;
+    Cur(const u8 *p_, u64 n_) : p(p_), n(n_) {}
+    int u8v(u8 &v)
+    {
+        if (pos >= n) return FQXV_NAMES_ERR_TRUNCATED;
+        v = p[pos++];
+        return 0;
+    }
+    int varint(u64 &v)
+    {
+        v = 0;
+        for (u32 shift = 0;; shift += 7) {
+            if (pos >= n || shift >= 64) return FQXV_NAMES_ERR_VARINT;
+            const u8 b = p[pos++];
+            v |= (u64)(b & 0x7f) << shift;
+            if (!(b & 0x80)) return 0;
+        }
+    }
+    int take(u64 len, u64 &at)
+    {
+        if (len > n - pos) return FQXV_NAMES_ERR_TRUNCATED;
+        at = pos;
+        pos += len;
+        return 0;
+    }
+};
+
+enum Kind : u8 { EV_SUB, EV_WIDTH, EV_ERR };
+
+struct Event {
+    Kind kind;
+    u8 col;
+    int err;       // EV_ERR: the framing error
+    u64 width;     // EV_WIDTH
+    u64 off, len;  // EV_SUB: sub-stream bytes, offset in the caller's buffer
+    u64 declared;  // peeked output length (when len >= 9)
+    i64 job;       // device job, or -1 when the host settles it
+    int host_st;   // status when job < 0
+};
+
+struct Stream {
+    std::vector<Event> ev;
+    u64 n_records = 0, max_out = 0;
+    // results
+    int status = 0;
+    std::vector<u8> rec;   // [u64 n][u32 len][bytes]...
+    u64 n_names = 0;
+};
+
+u64 peek_n(const u8 *s) { u64 v; memcpy(&v, s + 1, 8); return v; }
+
+// Phase 1: the framing walk (lib.rs:547-588 read order).
+void frame(const u8 *src, u64 base, u64 len, Stream &S)
+{
+    Cur r(src + base, len);
+    auto fail = [&](int e) { S.ev.push_back(Event{EV_ERR, 0, e, 0, 0, 0, 0, -1, 0}); };
+    u8 ver;
+    int e;
+    if ((e = r.u8v(ver))) return fail(e);
+    if (ver != FORMAT_VERSION) return fail(FQXV_NAMES_ERR_VERSION);
+    if ((e = r.varint(S.n_records))) return fail(e);
+    S.max_out = len > UINT64_MAX / MAX_STREAM_EXPANSION ? UINT64_MAX : len * MAX_STREAM_EXPANSION;
+    S.max_out = std::max(S.max_out, MIN_STREAM_BOUND);
+    auto sub = [&](u8 col) -> int {
+        u64 n, at;
+        int e2;
+        if ((e2 = r.varint(n)) || (e2 = r.take(n, at))) return e2;
+        Event ev{EV_SUB, col, 0, 0, base + at, n, 0, -1, 0};
+        if (n >= 9) ev.declared = peek_n(src + base + at);
+        S.ev.push_back(ev);
+        return 0;
+    };
+    for (u32 k = 0; k < NFLAT; k++)
+        if ((e = sub(0))) return fail(e);
+    for (u32 c = 0; c < NCOL; c++) {
+        u64 w;
+        if ((e = r.varint(w))) return fail(e);
+        S.ev.push_back(Event{EV_WIDTH, (u8)c, 0, w, 0, 0, 0, -1, 0});
+        for (u64 pl = 0; pl < w; pl++)
+            if ((e = sub((u8)c))) return fail(e);
+    }
+}
+
+// Decimal of an i64 (Rust's `to_string`), left-padded with '0' to `width`
+// (lib.rs:684-698); natural_digits (lib.rs:253-263).
+u32 natural_digits(i64 v)
+{
+    if (v <= 0) return 1;
+    u32 d = 0;
+    for (u64 x = (u64)v; x; x /= 10) d++;
+    return d;
+}
+
+void put_num(std::vector<u8> &out, i64 v, u64 width)
+{
+    char buf[24];
+    int k = 0;
+    u64 m = v < 0 ? (u64)0 - (u64)v : (u64)v;
+    do {
+        buf[k++] = (char)('0' + m % 10);
+        m /= 10;
+    } while (m);
+    if (v < 0) buf[k++] = '-';
+    if (width > (u64)k) out.insert(out.end(), width - k, (u8)'0');
+    for (int i = k - 1; i >= 0; i--) out.push_back((u8)buf[i]);
+}
+
+struct Tok {
+    bool is_num;
+    u64 off, len; // within the name
+    i64 value;
+};
+
+// Phase 3 for one stream.  `dec` is the device arena, `job_st`/`job_len`
+// the batch results, `job_off` each job's arena offset.
+int replay(Stream &S, const u8 *dec, const i32 *job_st, const u64 *job_len, const u64 *job_off)
+{
+    // Bytes of one decoded sub-stream, or its error, in the reference's
+    // order of checks (lib.rs:105-110 / fqxv-rans lib.rs:226-231).
+    auto bytes_of = [&](const Event &ev, u64 bound, const u8 *&p, u64 &n) -> int {
+        if (ev.len >= 9 && ev.declared > bound) return FQXV_RANS_ERR_BOUND;
+        const int st = ev.job >= 0 ? job_st[ev.job] : ev.host_st;
+        if (st) return st;
+        p = ev.job >= 0 ? dec + job_off[ev.job] : nullptr;
+        n = ev.job >= 0 ? job_len[ev.job] : 0;
+        return 0;
+    };
+    const u8 *fp[NFLAT] = {};
+    u64 fn[NFLAT] = {};
+    size_t e = 0;
+    const std::vector<Event> &ev = S.ev;
+    for (u32 k = 0; k < NFLAT; k++, e++) {
+        if (e >= ev.size()) return FQXV_NAMES_ERR_TRUNCATED; // unreachable: the walk ends in EV_ERR
+        if (ev[e].kind == EV_ERR) return ev[e].err;
+        int rc = bytes_of(ev[e], S.max_out, fp[k], fn[k]);
+        if (rc) return rc;
+        if (k == 0 && S.n_records > fn[0]) return FQXV_NAMES_ERR_RECORD_COUNT; // lib.rs:565-567
+    }
+    // Op run-length expansion (lib.rs:202-245).
+    std::vector<std::vector<u8>> ops(NCOL);
+    {
+        Cur cc(fp[5], fn[5]), cl(fp[7], fn[7]);
+        const u8 *syms = fp[6];
+        const u64 nsyms = fn[6];
+        u64 sym_pos = 0, total = 0;
+        for (u32 c = 0; c < NCOL; c++) {
+            u64 n_runs;
+            int rc = cc.varint(n_runs);
+            if (rc) return rc;
+            for (u64 i = 0; i < n_runs; i++) {
+                if (sym_pos >= nsyms) return FQXV_NAMES_ERR_OP_SYM_UNDERRUN;
+                const u8 s = syms[sym_pos++];
+                u64 run;
+                if ((rc = cl.varint(run))) return rc;
+                if (run > S.n_records || ops[c].size() + run > S.n_records) return FQXV_NAMES_ERR_OP_RUN_TOO_LONG;
+                if (run > S.max_out - total) return FQXV_NAMES_ERR_OP_BOUND;
+                total += run;
+                ops[c].insert(ops[c].end(), run, s);
+            }
+        }
+    }
+    u64 n_delta[NCOL];
+    for (u32 c = 0; c < NCOL; c++) n_delta[c] = (u64)std::count(ops[c].begin(), ops[c].end(), OP_DELTA);
+    // Delta byte planes (lib.rs:137-168).
+    std::vector<std::vector<u64>> deltas(NCOL);
+    for (u32 c = 0; c < NCOL; c++) {
+        if (e >= ev.size()) return FQXV_NAMES_ERR_TRUNCATED;
+        if (ev[e].kind == EV_ERR) return ev[e].err;
+        const u64 w = ev[e].width;
+        e++;
+        if (w == 0) continue;
+        deltas[c].assign(n_delta[c], 0);
+        for (u64 pl = 0; pl < w; pl++, e++) {
+            if (e >= ev.size()) return FQXV_NAMES_ERR_TRUNCATED;
+            if (ev[e].kind == EV_ERR) return ev[e].err;
+            const u8 *p;
+            u64 n;
+            int rc = bytes_of(ev[e], n_delta[c], p, n);
+            if (rc) return rc;
+            if (n != n_delta[c]) return FQXV_NAMES_ERR_PLANE_LEN;
+            const u32 sh = (u32)((8 * pl) & 63); // release-mode `<<` of lib.rs:163
+            for (u64 i = 0; i < n; i++) deltas[c][i] |= (u64)p[i] << sh;
+        }
+    }
+    // Rebuild (lib.rs:590-650).
+    Cur c_counts(fp[0], fn[0]), c_sl(fp[1], fn[1]), c_num(fp[3], fn[3]);
+    const u8 *str_data = fp[2], *pads = fp[4];
+    const u64 str_n = fn[2], pads_n = fn[4];
+    u64 str_pos = 0, pad_pos = 0;
+    u64 opos[NCOL] = {}, dpos[NCOL] = {};
+    std::vector<u8> &out = S.rec;
+    out.clear();
+    out.reserve(8 + std::min<u64>(S.n_records, 1ull << 20) * 40);
+    out.resize(8);
+    std::vector<Tok> prev, cur;
+    u64 prev_start = 0;
+    for (u64 r = 0; r < S.n_records; r++) {
+        u64 n_toks;
+        int rc = c_counts.varint(n_toks);
+        if (rc) return rc;
+        const u64 start = out.size() + 4;
+        out.resize(start);
+        cur.clear();
+        for (u64 t = 0; t < n_toks; t++) {
+            const u32 col = (u32)std::min<u64>(cur.size(), NCOL - 1);
+            if (opos[col] >= ops[col].size()) return FQXV_NAMES_ERR_TRUNCATED;
+            const u8 op = ops[col][opos[col]++];
+            const u64 here = out.size() - start;
+            if (op == OP_MATCH) {
+                if (cur.size() >= prev.size()) return FQXV_NAMES_ERR_MATCH;
+                Tok p = prev[cur.size()];
+                const u64 at = out.size();
+                out.resize(at + p.len);
+                memmove(out.data() + at, out.data() + prev_start + p.off, p.len);
+                p.off = here;
+                cur.push_back(p);
+            } else if (op == OP_STR) {
+                u64 ln;
+                if ((rc = c_sl.varint(ln))) return rc;
+                if (ln > str_n - str_pos) return FQXV_NAMES_ERR_STR_UNDERRUN;
+                out.insert(out.end(), str_data + str_pos, str_data + str_pos + ln);
+                str_pos += ln;
+                cur.push_back(Tok{false, here, ln, 0});
+            } else if (op == OP_NUM) {
+                u64 z;
+                if ((rc = c_num.varint(z))) return rc;
+                const i64 v = (i64)z;
+                if (pad_pos >= pads_n) return FQXV_NAMES_ERR_PAD_UNDERRUN;
+                const u64 width = natural_digits(v) + pads[pad_pos++];
+                put_num(out, v, width);
+                cur.push_back(Tok{true, here, out.size() - start - here, v});
+            } else if (op == OP_DELTA) {
+                if (dpos[col] >= deltas[col].size()) return FQXV_NAMES_ERR_DELTA_UNDERRUN;
+                const u64 z = deltas[col][dpos[col]++];
+                const i64 d = (i64)((z >> 1) ^ ((u64)0 - (z & 1)));
+                if (cur.size() >= prev.size() || !prev[cur.size()].is_num) return FQXV_NAMES_ERR_DELTA_PRIOR;
+                const i64 v = (i64)((u64)prev[cur.size()].value + (u64)d);
+                if (pad_pos >= pads_n) return FQXV_NAMES_ERR_PAD_UNDERRUN;
+                const u64 width = natural_digits(v) + pads[pad_pos++];
+                put_num(out, v, width);
+                cur.push_back(Tok{true, here, out.size() - start - here, v});
+            } else {
+                return FQXV_NAMES_ERR_UNKNOWN_OP;
+            }
+        }
+        const u32 nl = (u32)(out.size() - start); // `name.len() as u32`
+        memcpy(out.data() + start - 4, &nl, 4);
+        prev.swap(cur);
+        prev_start = start;
+    }
+    memcpy(out.data(), &S.n_records, 8);
+    S.n_names = S.n_records;
+    return 0;
+}
+
+} // namespace
+
+struct fqxv_names_batch {
+    std::vector<Stream> s;
+};
+
+extern "C" {
+
+static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint64_t *off, const uint64_t *len,
+                              uint32_t count, fqxv_names_batch **out)
+{
+    if (!out) return FQXV_RANS_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!ctx || (count && (!src || !off || !len))) return FQXV_RANS_ERR_INVALID_ARG;
+    std::unique_ptr<fqxv_names_batch> B(new fqxv_names_batch());
+    B->s.resize(count);
+    // Phase 1.
+    u64 lo = UINT64_MAX, hi = 0;
+    for (u32 i = 0; i < count; i++) {
+        frame(src, off[i], len[i], B->s[i]);
+        lo = std::min<u64>(lo, off[i]);
+        hi = std::max<u64>(hi, off[i] + len[i]);
+    }
+    if (count == 0) lo = 0;
+    // Jobs: every sub-stream of >= 9 bytes within its static bound (the
+    // flat streams' max_out; a plane's n_delta never exceeds n_records).
+    std::vector<fqxv_rans_job> jobs;
+    std::vector<u64> job_off;
+    u64 dst_total = 0;
+    for (Stream &S : B->s) {
+        bool plane_part = false;
+        for (Event &ev : S.ev) {
+            if (ev.kind == EV_WIDTH) plane_part = true;
+            if (ev.kind != EV_SUB) continue;
+            const u64 bound = plane_part ? S.n_records : S.max_out;
+            if (ev.len < 9) {
+                ev.host_st = FQXV_RANS_ERR_TRUNCATED; // scalar.rs: tag + u64 n need 9 bytes
+                continue;
+            }
+            if (ev.declared > bound) {
+                ev.host_st = FQXV_RANS_ERR_BOUND;
+                continue;
+            }
+            ev.job = (i64)jobs.size();
+            const u64 cap = (ev.declared + 255) & ~255ull;
+            jobs.push_back(fqxv_rans_job{ev.off - lo, ev.len, dst_total, ev.declared, UINT64_MAX, 0, 0});
+            job_off.push_back(dst_total);
+            dst_total += cap;
+        }
+    }
+    // Phase 2: one batched launch over all sub-streams.
+    std::vector<u8> dec(dst_total);
+    std::vector<i32> job_st(jobs.size());
+    std::vector<u64> job_len(jobs.size());
+    if (!jobs.empty()) {
+        const int rc = fqxv_decode_jobs_host(ctx, src + lo, hi - lo, jobs.data(), (u32)jobs.size(), dst_total,
+                                             dec.data(), job_st.data(), job_len.data());
+        if (rc) return rc;
+    }
+    // Phase 3, threads across streams.
+    const u32 hw = std::max(1u, std::thread::hardware_concurrency());
+    const u32 nthr = std::min<u32>({count, hw, 16u});
+    std::atomic<u32> next{0};
+    auto work = [&]() {
+        for (u32 i; (i = next.fetch_add(1)) < count;) {
+            Stream &S = B->s[i];
+            try {
+                S.status = replay(S, dec.data(), job_st.data(), job_len.data(), job_off.data());
+            } catch (const std::bad_alloc &) {
+                S.status = FQXV_NAMES_ERR_ALLOC; // the reference's fallible reserves (lib.rs:150-152, 244-245)
+            }
+            if (S.status) {
+                S.rec.clear();
+                S.n_names = 0;
+            }
+            S.ev.clear();
+            S.ev.shrink_to_fit();
+        }
+    };
+    if (nthr <= 1) {
+        work();
+    } else {
+        std::vector<std::thread> th;
+        for (u32 t = 0; t < nthr; t++) th.emplace_back(work);
+        for (auto &t : th) t.join();
+    }
+    *out = B.release();
+    return FQXV_RANS_OK;
+}
+
+int fqxv_names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint64_t *off, const uint64_t *len,
+                            uint32_t count, fqxv_names_batch **out)
+{
+    try {
+        return names_decode_batch(ctx, src, off, len, count, out);
+    } catch (const std::bad_alloc &) {
+        return FQXV_RANS_ERR_DEVICE;
+    }
+}
+
+int32_t fqxv_names_status(const fqxv_names_batch *b, uint32_t i)
+{
+    if (!b || i >= b->s.size()) return FQXV_RANS_ERR_INVALID_ARG;
+    return b->s[i].status;
+}
+
+int fqxv_names_records(const fqxv_names_batch *b, uint32_t i, const uint8_t **data, uint64_t *size,
+                       uint64_t *n_names)
+{
+    if (!b || i >= b->s.size() || !data || !size || !n_names) return FQXV_RANS_ERR_INVALID_ARG;
+    const Stream &S = b->s[i];
+    *data = S.rec.data();
+    *size = S.rec.size();
+    *n_names = S.n_names;
+    return S.status;
+}
+
+void fqxv_names_batch_destroy(fqxv_names_batch *b) { delete b; }
+
+} // extern "C"

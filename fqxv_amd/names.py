@@ -1,0 +1,108 @@
+"""Host-side mirror of the read-name tokenizer's decode over the gfx950 library.
+
+Reference surface (crates/fqxv-tokenizer/src/lib.rs):
+
+    pub enum Error { Malformed(&'static str), Rans(fqxv_rans::Error) }  lib.rs:40-49
+    pub fn decode(src: &[u8]) -> Result<Vec<Vec<u8>>>                  lib.rs:543-650
+
+``decode`` keeps that signature; ``decode_batch`` is the batched form the
+container's block loop would call (crates/fqxv/src/container/block.rs:1060,
+one names stream per block): the framing of every stream is walked on the
+host, all rANS sub-streams of all streams are decoded in ONE device launch
+(``fqxv_names_decode_batch``), and the names are rebuilt natively.  Errors
+carry the reference's messages; there is no CPU fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import struct
+
+import numpy as np
+
+from . import _lib
+from .rans import Coder, DeviceError, default_coder
+
+
+class Error(Exception):
+    """lib.rs:40-49.  ``str(e)`` is the reference's message (for a rANS
+    sub-stream failure, fqxv-rans's message)."""
+
+    def __init__(self, msg: str, code: int):
+        super().__init__(msg)
+        self.code = code
+
+
+class Malformed(Error):
+    """``Error::Malformed`` -- a framing or token-level check failed."""
+
+
+class Rans(Error):
+    """``Error::Rans`` -- a sub-stream failed in the entropy coder."""
+
+
+def _error(code: int) -> Error:
+    msg = _lib.strerror(code)
+    return Malformed(msg, code) if code >= _lib.NAMES_ERR_FIRST else Rans(msg, code)
+
+
+def parse_records(rec: bytes) -> list[bytes]:
+    """[u64 n] then n x [u32 len][bytes] -> the list of names."""
+    n = struct.unpack_from("<Q", rec, 0)[0]
+    out, p = [], 8
+    for _ in range(n):
+        ln = struct.unpack_from("<I", rec, p)[0]
+        out.append(rec[p + 4:p + 4 + ln])
+        p += 4 + ln
+    return out
+
+
+def decode_batch_records(streams, coder: Coder | None = None):
+    """Decode many names streams in one batched device pass.
+
+    Returns one entry per stream: the names in the digest layout of
+    crates/fqxv/src/container/block.rs:164-176 (``bytes``), or the
+    :class:`Error` the reference would return for that stream.
+    """
+    c = coder or default_coder()
+    L = c._L
+    parts = [bytes(s) for s in streams]
+    lens = np.array([len(s) for s in parts], dtype=np.uint64)
+    offs = np.zeros(len(parts), dtype=np.uint64)
+    if len(parts) > 1:
+        offs[1:] = np.cumsum(lens[:-1])
+    blob = b"".join(parts) or b"\0"
+    sbuf = ctypes.create_string_buffer(blob, len(blob))
+    h = ctypes.c_void_p()
+    rc = L.fqxv_names_decode_batch(c._h, sbuf, ctypes.c_void_p(offs.ctypes.data), ctypes.c_void_p(lens.ctypes.data),
+                                   len(parts), ctypes.byref(h))
+    if rc:
+        if rc == _lib.ERR_DEVICE:
+            raise DeviceError(_lib.strerror(rc))
+        raise ValueError(_lib.strerror(rc))
+    try:
+        out = []
+        for i in range(len(parts)):
+            p = ctypes.c_void_p()
+            size = ctypes.c_uint64()
+            n = ctypes.c_uint64()
+            st = L.fqxv_names_records(h, i, ctypes.byref(p), ctypes.byref(size), ctypes.byref(n))
+            if st:
+                out.append(_error(st))
+            else:
+                out.append(ctypes.string_at(p, size.value))
+        return out
+    finally:
+        L.fqxv_names_batch_destroy(h)
+
+
+def decode_batch(streams, coder: Coder | None = None):
+    """One list of names (or an :class:`Error`) per stream."""
+    return [r if isinstance(r, Error) else parse_records(r) for r in decode_batch_records(streams, coder)]
+
+
+def decode(src, coder: Coder | None = None) -> list[bytes]:
+    """lib.rs:543-650 ``decode(src)``."""
+    r = decode_batch([src], coder)[0]
+    if isinstance(r, Error):
+        raise r
+    return r

@@ -135,3 +135,65 @@ def workload(name: str, blocks: int | None = None) -> Corpus:
     else:
         raise KeyError(name)
     return c
+
+
+# ---- read-name lists for the names-stream path ------------------------------
+
+NAME_KINDS = ("illumina", "paired", "sra", "arbitrary", "highbit", "maxcol")
+
+
+def make_names(kind: str, n: int, seed: int) -> list[bytes]:
+    """Synthetic FASTQ read names of one style:
+
+    - ``illumina``: ``INST7:41:HFXXX:<lane>:<tile>:<x>:<y>``, tiles rising,
+      x/y coordinates drifting upward within a tile;
+    - ``paired``: the same, mates interleaved with `` 1:N:0:<barcode>`` /
+      `` 2:N:0:<barcode>`` comments (a G=2 block);
+    - ``sra``: ``SRR<run>.<i> <i> length=<len>`` (a counter column);
+    - ``arbitrary``: 0-40 bytes drawn from ``AB:._-0129 `` (the alphabet of
+      the tokenizer's own property tests);
+    - ``highbit``: arbitrary bytes 0x00-0xff;
+    - ``maxcol``: alternating letter/digit names of 58-64 single-byte tokens,
+      up to the 64-column cap.  (Past it, the last column collects every
+      further token, and the reference's decoder rejects a column holding
+      more ops than there are records: crates/fqxv-tokenizer/src/lib.rs:
+      232-237.)
+    """
+    r = np.random.default_rng(seed)
+    out: list[bytes] = []
+    if kind in ("illumina", "paired"):
+        m = n // 2 + 1 if kind == "paired" else n
+        tile, x, y = 1101, 1000, 2000
+        lane = int(r.integers(1, 9))
+        steps = r.integers(0, 40, size=(m, 2))
+        newtile = r.random(m) < 0.001
+        codes = [b"ACGTAC", b"TTAGGC", b"CGATGT"]
+        for i in range(m):
+            if newtile[i]:
+                tile += 1
+                x, y = 1000, 2000
+            x += int(steps[i, 0])
+            y += int(steps[i, 1]) * 3
+            base = b"INST7:41:HFXXX:%d:%d:%d:%d" % (lane, tile, x, y)
+            if kind == "paired":
+                bc = codes[i % 3]
+                out.append(base + b" 1:N:0:" + bc)
+                out.append(base + b" 2:N:0:" + bc)
+            else:
+                out.append(base)
+        return out[:n]
+    if kind == "sra":
+        run = int(r.integers(100000, 999999))
+        lens = r.choice([100, 101, 150, 151], size=n, p=[0.1, 0.2, 0.6, 0.1])
+        return [b"SRR%d.%d %d length=%d" % (run, i + 1, i + 1, int(lens[i])) for i in range(n)]
+    if kind == "arbitrary":
+        alpha = np.frombuffer(b"AB:._-0129 ", dtype=np.uint8)
+        ls = r.integers(0, 41, size=n)
+        return [alpha[r.integers(0, len(alpha), size=int(k))].tobytes() for k in ls]
+    if kind == "highbit":
+        ls = r.integers(0, 24, size=n)
+        return [r.integers(0, 256, size=int(k), dtype=np.int64).astype(np.uint8).tobytes() for k in ls]
+    if kind == "maxcol":
+        ls = r.integers(58, 65, size=n)
+        return [bytes(b"a1"[j % 2] for j in range(int(k))) for k in ls]
+    raise ValueError(kind)

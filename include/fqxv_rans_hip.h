@@ -46,6 +46,25 @@ extern "C" {
 #define FQXV_RANS_ERR_INVALID_ARG 9      /* bad order / NULL pointer / bad job                         */
 #define FQXV_RANS_ERR_DEVICE 10          /* HIP runtime failure (allocation, launch, copy)             */
 
+/* Names-stream errors: the reference's Error::Malformed strings of
+ * crates/fqxv-tokenizer/src/lib.rs (a sub-stream's rANS failure keeps its
+ * FQXV_RANS_ERR_* code). */
+#define FQXV_NAMES_ERR_TRUNCATED 32      /* "truncated"                          lib.rs:705        */
+#define FQXV_NAMES_ERR_VARINT 33         /* "varint too long"                    lib.rs:708        */
+#define FQXV_NAMES_ERR_VERSION 34        /* "unsupported version"                lib.rs:546        */
+#define FQXV_NAMES_ERR_RECORD_COUNT 35   /* "record count exceeds token stream"  lib.rs:566        */
+#define FQXV_NAMES_ERR_OP_SYM_UNDERRUN 36 /* "op sym underrun"                   lib.rs:226        */
+#define FQXV_NAMES_ERR_OP_RUN_TOO_LONG 37 /* "op run too long"                   lib.rs:237        */
+#define FQXV_NAMES_ERR_OP_BOUND 38       /* "op columns exceed the stream bound" lib.rs:242        */
+#define FQXV_NAMES_ERR_PLANE_LEN 39      /* "delta plane length mismatch"        lib.rs:160        */
+#define FQXV_NAMES_ERR_MATCH 40          /* "MATCH without prior token"          lib.rs:606        */
+#define FQXV_NAMES_ERR_STR_UNDERRUN 41   /* "string data underrun"               lib.rs:614        */
+#define FQXV_NAMES_ERR_PAD_UNDERRUN 42   /* "pad underrun"                       lib.rs:588        */
+#define FQXV_NAMES_ERR_DELTA_UNDERRUN 43 /* "delta underrun"                     lib.rs:631        */
+#define FQXV_NAMES_ERR_DELTA_PRIOR 44    /* "DELTA without numeric prior"        lib.rs:637        */
+#define FQXV_NAMES_ERR_UNKNOWN_OP 45     /* "unknown op"                         lib.rs:642        */
+#define FQXV_NAMES_ERR_ALLOC 46          /* "op run too large to allocate"       lib.rs:245    */
+
 /* Order tags (lib.rs:58-64; the stream's first byte, scalar.rs:25-26). */
 #define FQXV_RANS_ORDER0 0
 #define FQXV_RANS_ORDER1 1
@@ -220,6 +239,29 @@ int fqxv_rans_perm_join(const uint8_t *d_planes, uint64_t n, uint32_t *d_perm, v
  * runs on `stream`; the call returns when the checksums are written. */
 int fqxv_rans_crc32c_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
                            uint32_t count, uint32_t *d_crc, void *stream);
+
+/* ---- Names streams (crates/fqxv-tokenizer/src/lib.rs, format 1) --------
+ *
+ * lib.rs:543-650 `decode(src)` for `count` names streams at once; stream i
+ * is src[off[i], off[i] + len[i]) in host memory (the names section of a
+ * block payload, crates/fqxv/src/container/mod.rs:88-95).  The framing is
+ * walked on the host, every rANS sub-stream of every stream is decoded in
+ * one batched device launch, and the names are rebuilt on the host.  The
+ * return code reports argument and device failures only; each stream's own
+ * result is fqxv_names_status (FQXV_RANS_OK, an FQXV_NAMES_ERR_* code, or a
+ * sub-stream's FQXV_RANS_ERR_* code -- the first failing check in the
+ * reference's order).  Synchronous. */
+typedef struct fqxv_names_batch fqxv_names_batch;
+int fqxv_names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint64_t *off, const uint64_t *len,
+                            uint32_t count, fqxv_names_batch **out);
+int32_t fqxv_names_status(const fqxv_names_batch *b, uint32_t i);
+/* Stream i's names as [u64 n] then n x ([u32 len][len bytes]), little-endian:
+ * exactly the bytes the block's names digest covers
+ * (crates/fqxv/src/container/block.rs:164-176).  Valid until destroy.
+ * Returns the stream's status. */
+int fqxv_names_records(const fqxv_names_batch *b, uint32_t i, const uint8_t **data, uint64_t *size,
+                       uint64_t *n_names);
+void fqxv_names_batch_destroy(fqxv_names_batch *b);
 
 /* Host helper: job order, largest `sizes[i]` first (ties by index). */
 void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);

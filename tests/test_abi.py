@@ -13,7 +13,7 @@ from conftest import ROOT
 def header_functions():
     text = open(os.path.join(ROOT, "include", "fqxv_rans_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(fqxv_rans_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(fqxv_(?:rans|names)_\w+)\s*\(", text)))
 
 
 @pytest.fixture(scope="module")

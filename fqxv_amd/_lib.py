@@ -40,7 +40,7 @@ EXPORTS = [
     "fqxv_rans_encode_best_batch", "fqxv_rans_encode_best_bound",
     "fqxv_rans_perm_split", "fqxv_rans_perm_join", "fqxv_rans_crc32c_batch",
     "fqxv_names_decode_batch", "fqxv_names_status", "fqxv_names_records",
-    "fqxv_names_batch_destroy",
+    "fqxv_names_batch_destroy", "fqxv_names_digests", "fqxv_rans_xxh3_batch",
 ]
 
 
@@ -125,6 +125,8 @@ def load():
                                          ctypes.POINTER(ctypes.c_uint64)]
         L.fqxv_names_batch_destroy.argtypes = [vp]
         L.fqxv_names_batch_destroy.restype = None
+        L.fqxv_names_digests.argtypes = [vp, vp, vp]
+        L.fqxv_rans_xxh3_batch.argtypes = [vp, vp, vp, vp, u32, vp, vp]
         _lib = L
         return L
 

@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfqxv_rans_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["rans_decode.hip", "rans_encode.hip", "rans_aux.hip", "rans_crc.hip", "rans_api.cpp", "rans_server.cpp",
+SOURCES = ["rans_decode.hip", "rans_encode.hip", "rans_aux.hip", "rans_crc.hip", "rans_xxh3.hip", "rans_api.cpp", "rans_server.cpp",
            "rans_names.cpp"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",

@@ -37,6 +37,8 @@ void fqxv_crc_const_fill(void *);
 hipError_t fqxv_launch_crc(const u8 *, const u64 *, const u64 *, const u32 *, const u32 *, u32, const u32 *, u32,
                            const void *, u32 *, u32 *, u32, hipStream_t);
 hipError_t fqxv_launch_perm_join(const u8 *, u64, u32 *, hipStream_t);
+hipError_t fqxv_launch_xxh3(const u8 *, const u64 *, const u64 *, const u32 *, const u32 *, u32, const u32 *, u32,
+                            u64 *, u64 *, hipStream_t);
 
 namespace {
 
@@ -73,6 +75,8 @@ struct fqxv_rans_ctx {
     u64 crc_cap = 0;
     u8 *job_buf = nullptr;                  // host-staged batches: jobs, order, status, out_len
     u64 job_cap = 0;
+    u8 *xxh_buf = nullptr;                  // per-call xxh3 range/item arrays and block sums
+    u64 xxh_cap = 0;
 };
 
 namespace {
@@ -242,6 +246,7 @@ void fqxv_rans_ctx_destroy(fqxv_rans_ctx *c)
         if (c->crc_const) (void)hipFree(c->crc_const);
         if (c->crc_buf) (void)hipFree(c->crc_buf);
         if (c->job_buf) (void)hipFree(c->job_buf);
+        if (c->xxh_buf) (void)hipFree(c->xxh_buf);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -574,6 +579,79 @@ int fqxv_rans_crc32c_batch(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_
         return FQXV_RANS_ERR_DEVICE;
     // The staging vectors above are host locals: finish before returning.
     return hipStreamSynchronize(st) == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
+}
+
+// xxh3 of `count` ranges of a device arena; caller holds the context lock.
+static int xxh3_locked(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
+                       uint32_t count, uint64_t *d_out, hipStream_t st)
+{
+    // Block items (range, 1 KiB block) of the long inputs, in range order.
+    std::vector<u32> item_range, item_block, first(count);
+    for (u32 r = 0; r < count; r++) {
+        first[r] = (u32)item_range.size();
+        if (len[r] <= 240) continue;
+        const u64 nb = (len[r] - 1) >> 10;
+        if (item_range.size() + nb > 0xFFFFFFFFull) return FQXV_RANS_ERR_INVALID_ARG;
+        for (u64 k = 0; k < nb; k++) {
+            item_range.push_back(r);
+            item_block.push_back((u32)k);
+        }
+    }
+    const u64 ni = item_range.size();
+    // Device layout: sums[ni][8] off[count] len[count] (u64), first[count]
+    // item_range[ni] item_block[ni] (u32).
+    const u64 need = 64ull * ni + 16ull * count + 4ull * count + 8ull * ni + 64;
+    int rc = ensure_buf(&c->xxh_buf, &c->xxh_cap, need);
+    if (rc) return rc;
+    u64 *d_sums = (u64 *)c->xxh_buf, *d_off = d_sums + 8 * ni, *d_len = d_off + count;
+    u32 *d_first = (u32 *)(d_len + count), *d_ir = d_first + count, *d_ib = d_ir + ni;
+    if (hipMemcpyAsync(d_off, off, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_len, len, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_first, first.data(), 4ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (ni && hipMemcpyAsync(d_ir, item_range.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (ni && hipMemcpyAsync(d_ib, item_block.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess))
+        return FQXV_RANS_ERR_DEVICE;
+    if (fqxv_launch_xxh3(d_src, d_off, d_len, d_ir, d_ib, (u32)ni, d_first, count, d_sums, d_out, st) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    // The staging vectors above are host locals: finish before returning.
+    return hipStreamSynchronize(st) == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
+}
+
+int fqxv_rans_xxh3_batch(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
+                         uint32_t count, uint64_t *d_out, void *stream)
+{
+    if (!c || (count && (!d_src || !off || !len || !d_out))) return FQXV_RANS_ERR_INVALID_ARG;
+    if (count == 0) return FQXV_RANS_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    return xxh3_locked(c, d_src, off, len, count, d_out, pick_stream(c, stream));
+}
+
+// Host ranges: packed into the context's input buffer (256-byte aligned
+// starts), hashed, digests copied back.
+int fqxv_xxh3_host_ranges(fqxv_rans_ctx *c, const uint8_t *const *ptrs, const uint64_t *lens, uint32_t count,
+                          uint64_t *out)
+{
+    if (!c || (count && (!ptrs || !lens || !out))) return FQXV_RANS_ERR_INVALID_ARG;
+    if (count == 0) return FQXV_RANS_OK;
+    std::vector<u64> off(count);
+    u64 total = 0;
+    for (u32 i = 0; i < count; i++) {
+        off[i] = total;
+        total += (lens[i] + 255) & ~255ull;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    int rc;
+    if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, total)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, 8ull * count)))
+        return rc;
+    hipStream_t st = c->stream;
+    for (u32 i = 0; i < count; i++)
+        if (lens[i] && hipMemcpyAsync(c->d_in + off[i], ptrs[i], lens[i], hipMemcpyHostToDevice, st) != hipSuccess)
+            return FQXV_RANS_ERR_DEVICE;
+    if ((rc = xxh3_locked(c, c->d_in, off.data(), lens, count, (u64 *)c->d_out, st))) return rc;
+    if (hipMemcpy(out, c->d_out, 8ull * count, hipMemcpyDeviceToHost) != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+    return FQXV_RANS_OK;
 }
 
 int fqxv_rans_decode_one_device(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *job,

@@ -13,4 +13,9 @@ extern "C" {
 int fqxv_decode_jobs_host(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_len, const fqxv_rans_job *h_jobs,
                           uint32_t count, uint64_t dst_total, uint8_t *h_dst, int32_t *h_status,
                           uint64_t *h_out_len);
+
+// xxh3-64 of `count` host byte ranges on the device (uploaded, hashed,
+// digests copied to out).  Synchronous.
+int fqxv_xxh3_host_ranges(fqxv_rans_ctx *c, const uint8_t *const *ptrs, const uint64_t *lens, uint32_t count,
+                          uint64_t *out);
 }

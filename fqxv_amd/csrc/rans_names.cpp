@@ -422,6 +422,23 @@ int fqxv_names_records(const fqxv_names_batch *b, uint32_t i, const uint8_t **da
     return S.status;
 }
 
+int fqxv_names_digests(fqxv_rans_ctx *ctx, const fqxv_names_batch *b, uint64_t *out)
+{
+    if (!ctx || !b || (!out && !b->s.empty())) return FQXV_RANS_ERR_INVALID_ARG;
+    const u32 n = (u32)b->s.size();
+    std::vector<const u8 *> ptrs(n);
+    std::vector<u64> lens(n);
+    for (u32 i = 0; i < n; i++) {
+        ptrs[i] = b->s[i].rec.data();
+        lens[i] = b->s[i].status ? 0 : b->s[i].rec.size();
+    }
+    const int rc = fqxv_xxh3_host_ranges(ctx, ptrs.data(), lens.data(), n, out);
+    if (rc) return rc;
+    for (u32 i = 0; i < n; i++)
+        if (b->s[i].status) out[i] = 0;
+    return FQXV_RANS_OK;
+}
+
 void fqxv_names_batch_destroy(fqxv_names_batch *b) { delete b; }
 
 } // extern "C"

@@ -40,6 +40,12 @@ class Rans(Error):
     """``Error::Rans`` -- a sub-stream failed in the entropy coder."""
 
 
+class Corrupt(Exception):
+    """The container's ``Error::Corrupt { what: "block names digest" }``
+    (crates/fqxv/src/container/block.rs:1097-1109): the names decoded, but
+    not to the bytes the block's stored digest covers."""
+
+
 def _error(code: int) -> Error:
     msg = _lib.strerror(code)
     return Malformed(msg, code) if code >= _lib.NAMES_ERR_FIRST else Rans(msg, code)
@@ -56,12 +62,14 @@ def parse_records(rec: bytes) -> list[bytes]:
     return out
 
 
-def decode_batch_records(streams, coder: Coder | None = None):
+def decode_batch_records(streams, coder: Coder | None = None, digests: bool = False):
     """Decode many names streams in one batched device pass.
 
     Returns one entry per stream: the names in the digest layout of
     crates/fqxv/src/container/block.rs:164-176 (``bytes``), or the
-    :class:`Error` the reference would return for that stream.
+    :class:`Error` the reference would return for that stream.  With
+    ``digests=True`` returns ``(records, digests)``, the digests (xxh3-64 of
+    each stream's records, 0 for a failed stream) computed on the device.
     """
     c = coder or default_coder()
     L = c._L
@@ -90,19 +98,39 @@ def decode_batch_records(streams, coder: Coder | None = None):
                 out.append(_error(st))
             else:
                 out.append(ctypes.string_at(p, size.value))
-        return out
+        if not digests:
+            return out
+        dg = np.zeros(max(len(parts), 1), dtype=np.uint64)
+        rc = L.fqxv_names_digests(c._h, h, ctypes.c_void_p(dg.ctypes.data))
+        if rc:
+            raise DeviceError(_lib.strerror(rc))
+        return out, [int(x) for x in dg[: len(parts)]]
     finally:
         L.fqxv_names_batch_destroy(h)
 
 
-def decode_batch(streams, coder: Coder | None = None):
-    """One list of names (or an :class:`Error`) per stream."""
-    return [r if isinstance(r, Error) else parse_records(r) for r in decode_batch_records(streams, coder)]
+def decode_batch(streams, coder: Coder | None = None, expected_digests=None):
+    """One list of names (or an :class:`Error`) per stream.  With
+    ``expected_digests`` (one stored names digest per stream, or None to
+    skip one), each decoded stream is checked on the device as
+    block.rs:1097-1109 checks it; a mismatch gives :class:`Corrupt`."""
+    if expected_digests is None:
+        return [r if isinstance(r, Error) else parse_records(r) for r in decode_batch_records(streams, coder)]
+    recs, got = decode_batch_records(streams, coder, digests=True)
+    out = []
+    for r, d, want in zip(recs, got, expected_digests):
+        if isinstance(r, Error):
+            out.append(r)
+        elif want is not None and d != int(want):
+            out.append(Corrupt("block names digest"))
+        else:
+            out.append(parse_records(r))
+    return out
 
 
 def decode(src, coder: Coder | None = None) -> list[bytes]:
     """lib.rs:543-650 ``decode(src)``."""
     r = decode_batch([src], coder)[0]
-    if isinstance(r, Error):
+    if isinstance(r, (Error, Corrupt)):
         raise r
     return r

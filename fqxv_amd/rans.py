@@ -306,6 +306,19 @@ class Coder:
                                               len(offs), ctypes.c_void_p(out.data_ptr()), self._stream(stream)))
         return out[: len(offs)]
 
+    def xxh3_batch(self, src, offs, lens, stream=None):
+        """XXH3-64 (seed 0; the reference's xxhash-rust ``Xxh3``) of byte
+        ranges of a device arena; returns a device int64 tensor holding the
+        u64 digests' bits."""
+        import torch
+        offs = np.ascontiguousarray(offs, dtype=np.uint64)
+        lens = np.ascontiguousarray(lens, dtype=np.uint64)
+        out = torch.zeros(max(len(offs), 1), dtype=torch.int64, device=src.device)
+        _raise(self._L.fqxv_rans_xxh3_batch(self._h, ctypes.c_void_p(src.data_ptr()),
+                                            ctypes.c_void_p(offs.ctypes.data), ctypes.c_void_p(lens.ctypes.data),
+                                            len(offs), ctypes.c_void_p(out.data_ptr()), self._stream(stream)))
+        return out[: len(offs)]
+
     def set_grid(self, waves: int) -> None:
         """Persistent-grid size of later batched calls (0 = occupancy max)."""
         _raise(self._L.fqxv_rans_ctx_set_grid(self._h, int(waves)))

@@ -262,6 +262,20 @@ int32_t fqxv_names_status(const fqxv_names_batch *b, uint32_t i);
 int fqxv_names_records(const fqxv_names_batch *b, uint32_t i, const uint8_t **data, uint64_t *size,
                        uint64_t *n_names);
 void fqxv_names_batch_destroy(fqxv_names_batch *b);
+/* The names content digest of every stream of a decoded batch: xxh3-64 over
+ * its records (the value block.rs:164-176 stores and block.rs:1097-1109
+ * compares), computed on the device.  out[i] = 0 for a failed stream. */
+int fqxv_names_digests(fqxv_rans_ctx *ctx, const fqxv_names_batch *b, uint64_t *out);
+
+/* XXH3-64 (seed 0; the XXH3_64bits of xxHash 0.8 that the reference's
+ * xxhash-rust 0.8.18 `Xxh3` computes) of `count` byte ranges
+ * [off[i], off[i] + len[i]) of the device arena `d_src`: the container's
+ * per-stream content digests (crates/fqxv/src/container/block.rs:164-176,
+ * mod.rs:85-87) and the reorder layout's output digest (reorder.rs:1580).
+ * `off` and `len` are host arrays; d_out[i] (device) receives the digest.
+ * Work runs on `stream`; the call returns when the digests are written. */
+int fqxv_rans_xxh3_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
+                         uint32_t count, uint64_t *d_out, void *stream);
 
 /* Host helper: job order, largest `sizes[i]` first (ties by index). */
 void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);

@@ -41,6 +41,7 @@ EXPORTS = [
     "fqxv_rans_perm_split", "fqxv_rans_perm_join", "fqxv_rans_crc32c_batch",
     "fqxv_names_decode_batch", "fqxv_names_status", "fqxv_names_records",
     "fqxv_names_batch_destroy", "fqxv_names_digests", "fqxv_rans_xxh3_batch",
+    "fqxv_names_encode_batch", "fqxv_names_output",
 ]
 
 
@@ -126,6 +127,8 @@ def load():
         L.fqxv_names_batch_destroy.argtypes = [vp]
         L.fqxv_names_batch_destroy.restype = None
         L.fqxv_names_digests.argtypes = [vp, vp, vp]
+        L.fqxv_names_encode_batch.argtypes = [vp, vp, vp, vp, u32, ctypes.POINTER(vp)]
+        L.fqxv_names_output.argtypes = [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64)]
         L.fqxv_rans_xxh3_batch.argtypes = [vp, vp, vp, vp, u32, vp, vp]
         _lib = L
         return L

@@ -367,13 +367,15 @@ static int batch_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, 
 }
 
 // Host-staged batch: the source span and the jobs go up, one persistent
-// decode launch runs (largest output first), status, lengths and the whole
-// destination arena come back.  Serves the names decoder.
-int fqxv_decode_jobs_host(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_len, const fqxv_rans_job *h_jobs,
-                          uint32_t count, uint64_t dst_total, uint8_t *h_dst, int32_t *h_status,
-                          uint64_t *h_out_len)
+// launch runs (largest job first), status, lengths and the whole destination
+// arena come back.  mode 0 decodes, 2 encodes at both orders keeping the
+// smaller stream.  Serves the names decoder and encoder.
+int fqxv_run_jobs_host(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_t src_len,
+                       const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total, uint8_t *h_dst,
+                       int32_t *h_status, uint64_t *h_out_len)
 {
-    if (!c || (src_len && !h_src) || (count && (!h_jobs || !h_status || !h_out_len)) || (dst_total && !h_dst))
+    if (!c || (mode != 0 && mode != 2) || (src_len && !h_src) || (count && (!h_jobs || !h_status || !h_out_len)) ||
+        (dst_total && !h_dst))
         return FQXV_RANS_ERR_INVALID_ARG;
     if (count == 0) return FQXV_RANS_OK;
     std::vector<u64> cost(count);
@@ -386,7 +388,7 @@ int fqxv_decode_jobs_host(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_l
     fqxv_rans_lpt_order(cost.data(), count, order.data());
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard g(c->device);
-    const u32 grid = std::min<u32>(count, c->dec_waves);
+    const u32 grid = std::min<u32>(count, mode ? c->enc_waves : c->dec_waves);
     const u64 jb = sizeof(fqxv_rans_job) * (u64)count, ob = 4ull * count, sb = 4ull * count, lb = 8ull * count;
     int rc;
     if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, src_len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dst_total)) ||
@@ -401,9 +403,12 @@ int fqxv_decode_jobs_host(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_l
         hipMemcpyAsync(d_jobs, h_jobs, jb, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(d_order, order.data(), ob, hipMemcpyHostToDevice, st) != hipSuccess)
         return FQXV_RANS_ERR_DEVICE;
-    if (fqxv_launch_decode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len, c->ws,
-                                 grid, st) != hipSuccess)
-        return FQXV_RANS_ERR_DEVICE;
+    const hipError_t e =
+        mode ? fqxv_launch_encode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len,
+                                        c->ws, grid, st, 1)
+             : fqxv_launch_decode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len,
+                                        c->ws, grid, st);
+    if (e != hipSuccess) return FQXV_RANS_ERR_DEVICE;
     if (hipMemcpyAsync(h_status, d_st, sb, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(h_out_len, d_len, lb, hipMemcpyDeviceToHost, st) != hipSuccess ||
         (dst_total && hipMemcpyAsync(h_dst, c->d_out, dst_total, hipMemcpyDeviceToHost, st) != hipSuccess) ||

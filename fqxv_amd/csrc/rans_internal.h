@@ -6,13 +6,15 @@
 #include "../../include/fqxv_rans_hip.h"
 
 extern "C" {
-// Decodes `count` jobs whose streams lie in the host buffer h_src[0, src_len)
-// into a destination arena of dst_total bytes, in one batched device launch;
-// copies the arena to h_dst and the per-job status / output length back.
-// Synchronous.  Job offsets index h_src and the arena.
-int fqxv_decode_jobs_host(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_len, const fqxv_rans_job *h_jobs,
-                          uint32_t count, uint64_t dst_total, uint8_t *h_dst, int32_t *h_status,
-                          uint64_t *h_out_len);
+// Runs `count` jobs whose sources lie in the host buffer h_src[0, src_len)
+// against a destination arena of dst_total bytes, in one batched device
+// launch -- mode 0 decodes, mode 2 encodes at both orders keeping the
+// smaller stream (job.order = the order kept on a tie) -- and copies the
+// arena to h_dst and the per-job status / output length back.  Synchronous.
+// Job offsets index h_src and the arena.
+int fqxv_run_jobs_host(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_t src_len,
+                       const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total, uint8_t *h_dst,
+                       int32_t *h_status, uint64_t *h_out_len);
 
 // xxh3-64 of `count` host byte ranges on the device (uploaded, hashed,
 // digests copied to out).  Synchronous.

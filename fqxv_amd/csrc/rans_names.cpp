@@ -305,10 +305,154 @@ int replay(Stream &S, const u8 *dec, const i32 *job_st, const u64 *job_len, cons
     return 0;
 }
 
+
+// ---- encode (lib.rs:88-135, 175-200, 455-540, 654-681) --------------------
+
+void put_varint(std::vector<u8> &out, u64 v)
+{
+    while (v >= 0x80) {
+        out.push_back((u8)(v | 0x80));
+        v >>= 7;
+    }
+    out.push_back((u8)v);
+}
+
+struct TokE {
+    bool is_num;
+    const u8 *p;
+    u64 len;
+    i64 value;
+};
+
+// Maximal digit / non-digit runs; a digit run of at most 18 bytes is
+// numeric (lib.rs:654-681).
+void tokenize(const u8 *name, u64 n, std::vector<TokE> &out)
+{
+    out.clear();
+    u64 i = 0;
+    while (i < n) {
+        const bool dig = name[i] >= '0' && name[i] <= '9';
+        const u64 s = i;
+        while (i < n && (name[i] >= '0' && name[i] <= '9') == dig) i++;
+        const u64 len = i - s;
+        const bool is_num = dig && len <= 18;
+        i64 v = 0;
+        if (is_num)
+            for (u64 k = s; k < i; k++) v = v * 10 + (name[k] - '0');
+        out.push_back(TokE{is_num, name + s, len, v});
+    }
+}
+
+// The raw sub-streams of one block: the eight flat streams (counts,
+// str_lens, str_data, num_vals, pads, op run counts, op symbols, op run
+// lengths) and each column's delta byte planes.
+struct Parts {
+    u64 n = 0;
+    std::vector<u8> flat[NFLAT];
+    std::vector<std::vector<u8>> planes[NCOL];
+    int status = 0;
+    std::vector<u8> out; // the names stream
+};
+
+int encode_parts(const u8 *rec, u64 size, Parts &P)
+{
+    if (size < 8) return FQXV_RANS_ERR_INVALID_ARG;
+    memcpy(&P.n, rec, 8);
+    std::vector<u8> &counts = P.flat[0], &str_lens = P.flat[1], &str_data = P.flat[2], &num_vals = P.flat[3],
+                    &pads = P.flat[4];
+    std::vector<u8> op_cols[NCOL];
+    std::vector<u64> delta_vals[NCOL];
+    std::vector<TokE> prev, cur;
+    u64 pos = 8;
+    for (u64 r = 0; r < P.n; r++) {
+        u32 nl;
+        if (size - pos < 4) return FQXV_RANS_ERR_INVALID_ARG;
+        memcpy(&nl, rec + pos, 4);
+        pos += 4;
+        if (size - pos < nl) return FQXV_RANS_ERR_INVALID_ARG;
+        tokenize(rec + pos, nl, cur);
+        pos += nl;
+        for (u64 i = 0; i < cur.size(); i++) {
+            const TokE &t = cur[i];
+            const u32 col = (u32)std::min<u64>(i, NCOL - 1);
+            const TokE *p = i < prev.size() ? &prev[i] : nullptr;
+            if (p && p->len == t.len && memcmp(p->p, t.p, t.len) == 0) {
+                op_cols[col].push_back(OP_MATCH);
+            } else if (t.is_num) {
+                const u64 pad = t.len - natural_digits(t.value);
+                if (p && p->is_num) {
+                    op_cols[col].push_back(OP_DELTA);
+                    const i64 d = t.value - p->value;
+                    delta_vals[col].push_back(((u64)d << 1) ^ (u64)(d >> 63));
+                } else {
+                    op_cols[col].push_back(OP_NUM);
+                    put_varint(num_vals, (u64)t.value);
+                }
+                pads.push_back((u8)pad);
+            } else {
+                op_cols[col].push_back(OP_STR);
+                put_varint(str_lens, t.len);
+                str_data.insert(str_data.end(), t.p, t.p + t.len);
+            }
+        }
+        put_varint(counts, cur.size());
+        prev.swap(cur);
+    }
+    if (pos != size) return FQXV_RANS_ERR_INVALID_ARG;
+    // Op run lengths (lib.rs:175-200).
+    for (u32 c = 0; c < NCOL; c++) {
+        const std::vector<u8> &col = op_cols[c];
+        u64 runs = 0;
+        for (u64 i = 0; i < col.size();) {
+            u64 run = 1;
+            while (i + run < col.size() && col[i + run] == col[i]) run++;
+            P.flat[6].push_back(col[i]);
+            put_varint(P.flat[7], run);
+            runs++;
+            i += run;
+        }
+        put_varint(P.flat[5], runs);
+    }
+    // Delta byte planes (lib.rs:118-135).
+    for (u32 c = 0; c < NCOL; c++) {
+        const std::vector<u64> &v = delta_vals[c];
+        if (v.empty()) continue;
+        const u64 maxv = *std::max_element(v.begin(), v.end());
+        u32 bits = 0;
+        for (u64 x = maxv; x; x >>= 1) bits++;
+        const u32 width = std::max(1u, (bits + 7) / 8);
+        P.planes[c].resize(width);
+        for (u32 k = 0; k < width; k++) {
+            std::vector<u8> &pl = P.planes[c][k];
+            pl.resize(v.size());
+            for (u64 i = 0; i < v.size(); i++) pl[i] = (u8)(v[i] >> (8 * k));
+        }
+    }
+    return 0;
+}
+
+template <class F> void parallel_for(u32 count, F f)
+{
+    const u32 hw = std::max(1u, std::thread::hardware_concurrency());
+    const u32 nthr = std::min<u32>({count, hw, 16u});
+    std::atomic<u32> next{0};
+    auto work = [&]() {
+        for (u32 i; (i = next.fetch_add(1)) < count;) f(i);
+    };
+    if (nthr <= 1) {
+        work();
+        return;
+    }
+    std::vector<std::thread> th;
+    for (u32 t = 0; t < nthr; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
+}
+
 } // namespace
 
 struct fqxv_names_batch {
-    std::vector<Stream> s;
+    std::vector<Stream> s;   // decode results
+    std::vector<Parts> e;    // encode results
 };
 
 extern "C" {
@@ -360,8 +504,8 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
     std::vector<i32> job_st(jobs.size());
     std::vector<u64> job_len(jobs.size());
     if (!jobs.empty()) {
-        const int rc = fqxv_decode_jobs_host(ctx, src + lo, hi - lo, jobs.data(), (u32)jobs.size(), dst_total,
-                                             dec.data(), job_st.data(), job_len.data());
+        const int rc = fqxv_run_jobs_host(ctx, 0, src + lo, hi - lo, jobs.data(), (u32)jobs.size(), dst_total,
+                                          dec.data(), job_st.data(), job_len.data());
         if (rc) return rc;
     }
     // Phase 3, threads across streams.
@@ -405,10 +549,125 @@ int fqxv_names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint64
     }
 }
 
+static int names_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint64_t *off, const uint64_t *len,
+                              uint32_t count, fqxv_names_batch **out)
+{
+    std::unique_ptr<fqxv_names_batch> B(new fqxv_names_batch());
+    B->e.resize(count);
+    // Tokenize every block (host, threads across blocks).
+    parallel_for(count, [&](u32 i) {
+        try {
+            B->e[i].status = encode_parts(src + off[i], len[i], B->e[i]);
+        } catch (const std::bad_alloc &) {
+            B->e[i].status = FQXV_RANS_ERR_DEVICE;
+        }
+    });
+    // One encode-best job per sub-stream of every block (order-0 on a tie,
+    // lib.rs:88-95).
+    std::vector<fqxv_rans_job> jobs;
+    u64 src_total = 0, dst_total = 0;
+    for (Parts &P : B->e) {
+        if (P.status) continue;
+        auto add = [&](const std::vector<u8> &st) {
+            const u64 n = st.size();
+            jobs.push_back(fqxv_rans_job{src_total, n, dst_total, fqxv_rans_encode_best_bound(n), 0, 0, 0});
+            src_total += (n + 15) & ~15ull;
+            dst_total += (fqxv_rans_encode_best_bound(n) + 255) & ~255ull;
+        };
+        for (u32 k = 0; k < NFLAT; k++) add(P.flat[k]);
+        for (u32 c = 0; c < NCOL; c++)
+            for (const auto &pl : P.planes[c]) add(pl);
+    }
+    std::vector<u8> raw(src_total), enc(dst_total);
+    {
+        size_t j = 0;
+        for (Parts &P : B->e) {
+            if (P.status) continue;
+            auto put = [&](const std::vector<u8> &st) {
+                if (!st.empty()) memcpy(raw.data() + jobs[j].src_off, st.data(), st.size());
+                j++;
+            };
+            for (u32 k = 0; k < NFLAT; k++) put(P.flat[k]);
+            for (u32 c = 0; c < NCOL; c++)
+                for (const auto &pl : P.planes[c]) put(pl);
+        }
+    }
+    std::vector<i32> st(jobs.size());
+    std::vector<u64> ol(jobs.size());
+    if (!jobs.empty()) {
+        const int rc = fqxv_run_jobs_host(ctx, 2, raw.data(), src_total, jobs.data(), (u32)jobs.size(), dst_total,
+                                          enc.data(), st.data(), ol.data());
+        if (rc) return rc;
+    }
+    // Frame every block (lib.rs:515-537).
+    std::vector<size_t> first(count);
+    {
+        size_t j = 0;
+        for (u32 i = 0; i < count; i++) {
+            first[i] = j;
+            Parts &P = B->e[i];
+            if (P.status) continue;
+            j += NFLAT;
+            for (u32 c = 0; c < NCOL; c++) j += P.planes[c].size();
+        }
+    }
+    parallel_for(count, [&](u32 i) {
+        Parts &P = B->e[i];
+        if (P.status) return;
+        size_t j = first[i];
+        std::vector<u8> &o = P.out;
+        o.push_back(FORMAT_VERSION);
+        put_varint(o, P.n);
+        auto emit = [&]() {
+            if (st[j]) {
+                P.status = st[j];
+                return;
+            }
+            put_varint(o, ol[j]);
+            o.insert(o.end(), enc.data() + jobs[j].dst_off, enc.data() + jobs[j].dst_off + ol[j]);
+            j++;
+        };
+        for (u32 k = 0; k < NFLAT && !P.status; k++) emit();
+        for (u32 c = 0; c < NCOL && !P.status; c++) {
+            put_varint(o, P.planes[c].size());
+            for (size_t k = 0; k < P.planes[c].size() && !P.status; k++) emit();
+        }
+        for (auto &f : P.flat) std::vector<u8>().swap(f);
+        for (auto &c : P.planes) std::vector<std::vector<u8>>().swap(c);
+        if (P.status) o.clear();
+    });
+    *out = B.release();
+    return FQXV_RANS_OK;
+}
+
+int fqxv_names_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint64_t *off, const uint64_t *len,
+                            uint32_t count, fqxv_names_batch **out)
+{
+    if (!out) return FQXV_RANS_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!ctx || (count && (!src || !off || !len))) return FQXV_RANS_ERR_INVALID_ARG;
+    try {
+        return names_encode_batch(ctx, src, off, len, count, out);
+    } catch (const std::bad_alloc &) {
+        return FQXV_RANS_ERR_DEVICE;
+    }
+}
+
+int fqxv_names_output(const fqxv_names_batch *b, uint32_t i, const uint8_t **data, uint64_t *size)
+{
+    if (!b || i >= b->e.size() || !data || !size) return FQXV_RANS_ERR_INVALID_ARG;
+    const Parts &P = b->e[i];
+    *data = P.out.data();
+    *size = P.out.size();
+    return P.status;
+}
+
 int32_t fqxv_names_status(const fqxv_names_batch *b, uint32_t i)
 {
-    if (!b || i >= b->s.size()) return FQXV_RANS_ERR_INVALID_ARG;
-    return b->s[i].status;
+    if (!b) return FQXV_RANS_ERR_INVALID_ARG;
+    if (i < b->s.size()) return b->s[i].status;
+    if (i < b->e.size()) return b->e[i].status;
+    return FQXV_RANS_ERR_INVALID_ARG;
 }
 
 int fqxv_names_records(const fqxv_names_batch *b, uint32_t i, const uint8_t **data, uint64_t *size,

@@ -128,6 +128,64 @@ def decode_batch(streams, coder: Coder | None = None, expected_digests=None):
     return out
 
 
+def records(names) -> bytes:
+    """The records layout ([u64 n] then per name [u32 len][bytes]) of a
+    list of names: the input of :func:`encode_batch`."""
+    out = [struct.pack("<Q", len(names))]
+    for nm in names:
+        b = bytes(nm)
+        out.append(struct.pack("<I", len(b)))
+        out.append(b)
+    return b"".join(out)
+
+
+def encode_batch_records(recs, coder: Coder | None = None):
+    """lib.rs:455-540 ``encode`` of many blocks given as records buffers:
+    host tokenizing, every sub-stream of every block encoded at both orders
+    in one device launch (the smaller kept, order-0 on a tie), host framing.
+    Returns one names stream (``bytes``) per block."""
+    c = coder or default_coder()
+    L = c._L
+    parts = [bytes(r) for r in recs]
+    lens = np.array([len(p) for p in parts], dtype=np.uint64)
+    offs = np.zeros(len(parts), dtype=np.uint64)
+    if len(parts) > 1:
+        offs[1:] = np.cumsum(lens[:-1])
+    blob = b"".join(parts) or b"\0"
+    sbuf = ctypes.create_string_buffer(blob, len(blob))
+    h = ctypes.c_void_p()
+    rc = L.fqxv_names_encode_batch(c._h, sbuf, ctypes.c_void_p(offs.ctypes.data), ctypes.c_void_p(lens.ctypes.data),
+                                   len(parts), ctypes.byref(h))
+    if rc:
+        if rc == _lib.ERR_DEVICE:
+            raise DeviceError(_lib.strerror(rc))
+        raise ValueError(_lib.strerror(rc))
+    try:
+        out = []
+        for i in range(len(parts)):
+            p = ctypes.c_void_p()
+            size = ctypes.c_uint64()
+            st = L.fqxv_names_output(h, i, ctypes.byref(p), ctypes.byref(size))
+            if st == _lib.ERR_DEVICE:
+                raise DeviceError(_lib.strerror(st))
+            if st:
+                raise ValueError(f"block {i}: {_lib.strerror(st)}")
+            out.append(ctypes.string_at(p, size.value))
+        return out
+    finally:
+        L.fqxv_names_batch_destroy(h)
+
+
+def encode_batch(lists, coder: Coder | None = None) -> list[bytes]:
+    """One names stream per list of names."""
+    return encode_batch_records([records(n) for n in lists], coder)
+
+
+def encode(names, coder: Coder | None = None) -> bytes:
+    """lib.rs:455-540 ``encode(names)``."""
+    return encode_batch([names], coder)[0]
+
+
 def decode(src, coder: Coder | None = None) -> list[bytes]:
     """lib.rs:543-650 ``decode(src)``."""
     r = decode_batch([src], coder)[0]

@@ -262,6 +262,20 @@ int32_t fqxv_names_status(const fqxv_names_batch *b, uint32_t i);
 int fqxv_names_records(const fqxv_names_batch *b, uint32_t i, const uint8_t **data, uint64_t *size,
                        uint64_t *n_names);
 void fqxv_names_batch_destroy(fqxv_names_batch *b);
+/* lib.rs:455-540 `encode(names)` for `count` blocks at once: block i's
+ * names, in the records layout of fqxv_names_records, are
+ * src[off[i], off[i] + len[i]) in host memory.  Tokenizing runs on the
+ * host (threads across blocks); every sub-stream of every block is encoded
+ * at both orders in one batched device launch, the smaller kept (order-0 on
+ * a tie, lib.rs:88-95) -- the reference encodes each of them twice on the
+ * CPU; the framing is written on the host.  Streams come back through
+ * fqxv_names_output (byte-identical to the reference's encode).  A block
+ * whose records are malformed gets FQXV_RANS_ERR_INVALID_ARG.
+ * Synchronous. */
+int fqxv_names_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint64_t *off, const uint64_t *len,
+                            uint32_t count, fqxv_names_batch **out);
+/* Block i's names stream of an encode batch; returns the block's status. */
+int fqxv_names_output(const fqxv_names_batch *b, uint32_t i, const uint8_t **data, uint64_t *size);
 /* The names content digest of every stream of a decoded batch: xxh3-64 over
  * its records (the value block.rs:164-176 stores and block.rs:1097-1109
  * compares), computed on the device.  out[i] = 0 for a failed stream. */

@@ -211,3 +211,34 @@ def test_structured_mutations_match_oracle(nm, no, coder):
         kinds.add(e[1] if isinstance(e, tuple) else "ok")
         assert got(nm, r) == e, m.hex()
     assert len(kinds) >= 8, kinds
+
+
+def test_encode_matches_oracle(nm, no, coder):
+    """fqxv_names_encode_batch: host tokenizing + one batched encode-best
+    launch + host framing gives the reference's bytes."""
+    from fqxv_amd import synth
+    from test_names_oracle import REF_LISTS
+    lists = list(REF_LISTS) + [synth.make_names(k, n, 40 + n) for k in synth.NAME_KINDS for n in (0, 1, 9, 800)]
+    lists.append([bytes(b"a1"[j % 2] for j in range(70))] * 3)  # past the column cap: still encodes
+    out = nm.encode_batch(lists, coder)
+    for names, s in zip(lists, out):
+        assert s == no.encode(names)
+
+
+def test_encode_reproduces_fixture_streams(nm, no, coder):
+    """Re-encoding each golden block's names on the device gives the stored
+    names stream byte for byte."""
+    streams = [s for _, s in fixture_names_streams()]
+    names = nm.decode_batch(streams, coder)
+    assert nm.encode_batch(names, coder) == streams
+
+
+def test_encode_decode_large_batch(nm, no, coder):
+    from fqxv_amd import synth
+    lists = [synth.make_names(k, 30000, 60 + i) for i, k in enumerate(("illumina", "paired", "sra", "illumina"))]
+    streams = nm.encode_batch(lists, coder)
+    assert streams[2] == no.encode(lists[2])
+    recs, dg = nm.decode_batch_records(streams, coder, digests=True)
+    for names, r, d in zip(lists, recs, dg):
+        assert nm.parse_records(r) == names
+        assert d == no.names_digest(names)

@@ -221,12 +221,14 @@ def bench_c4(args, coder, dev, oracle):
             f"p50/p99 {pb['p50_block_ms']}/{pb['p99_block_ms']} ms")
         torch.cuda.synchronize()
         assert torch.equal(out[:n_total], arena[:n_total])
+    xxh = bench_xxh3(coder, arena, corpus.offs, corpus.sizes, args.workload) if args.crc else None
 
     # CPU-baseline sample (host copies of a subset of the streams).
     cpu_idx = np.arange(0, nb, args.cpu_every)
     host_c, host_off = slices_to_host(enc.dst, enc.dst_off[cpu_idx], comp_len[cpu_idx])
     res = {
         "crc32c": crc,
+        "xxh3": xxh,
         "encode_best": best,
         "corpus": corpus, "value": value, "t_step": t_step, "t_kern": t_kern, "n_total": n_total,
         "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "perblock": pb,
@@ -410,6 +412,80 @@ def bench_crc(coder, oracle, enc, comp_len, corpus_name):
                       "when the checksums are written)", **res}
 
 
+def bench_xxh3(coder, arena, offs, lens, corpus_name):
+    """Content digests (XXH3-64, block.rs:164-176) on the device over the
+    decoded blocks, one range each, and over the whole arena as one range.
+    The block-sum kernel is HBM-bound (1 KiB read per 64 B of lane sums);
+    timing is host wall clock per call, which includes the host-side plan
+    and returns when the digests are written."""
+    import torch
+    import xxhash
+    offs = np.asarray(offs, dtype=np.uint64)
+    lens = np.asarray(lens, dtype=np.uint64)
+    got = coder.xxh3_batch(arena, offs, lens).cpu().numpy().view(np.uint64)
+    idx = list(range(0, len(offs), max(1, len(offs) // 8)))
+    for j in idx:
+        o, n = int(offs[j]), int(lens[j])
+        assert int(got[j]) == xxhash.xxh3_64_intdigest(arena[o:o + n].cpu().numpy().tobytes()), j
+    res = {}
+    total = int(lens.sum())
+    # One range the size of a block's largest decoded sequence stream
+    # (MAX_BLOCK_SEQ_BYTES = 256 MiB, crates/fqxv/src/container/format.rs:11):
+    # the digest's scramble chain is serial over 1 KiB blocks, so a single
+    # range is latency-bound.
+    one = min(256 << 20, int(offs[-1] + lens[-1]))
+    for name, (o, l, nbytes) in {"per_block": (offs, lens, total),
+                                 "one_256mib_range": (np.zeros(1, np.uint64), np.array([one], np.uint64), one)}.items():
+        coder.xxh3_batch(arena, o, l)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            coder.xxh3_batch(arena, o, l)
+        t = (time.perf_counter() - t0) / reps
+        res[name] = {"ranges": len(o), "bytes": nbytes, "gbs": round(nbytes / t / 1e9, 1),
+                     "roofline_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 3)}
+    log(f"[xxh3] per-block {res['per_block']['gbs']} GB/s, one 256 MiB range {res['one_256mib_range']['gbs']} GB/s")
+    return {"corpus": f"decoded blocks of {corpus_name}", "checked_vs_xxhash": len(idx),
+            "timing": "host wall clock per call", **res}
+
+
+# ---------------------------------------------------------------------- names
+def bench_names(coder, blocks=32, per_block=40000, distinct=8):
+    """Read-name streams (crates/fqxv-tokenizer): ``blocks`` blocks of
+    ``per_block`` names (``distinct`` generated lists -- paired Illumina,
+    Illumina, SRA -- repeated), encoded and decoded through the batched
+    device path from host buffers (framing and tokenizing on the host, every
+    sub-stream of every block in one device launch), then the names digests
+    on the device.  Wall clock per call, PCIe transfers included."""
+    from fqxv_amd import names, synth
+    kinds = ("paired", "illumina", "sra")
+    lists = [synth.make_names(kinds[i % 3], per_block, 500 + i) for i in range(distinct)]
+    recs = [names.records(lists[i % distinct]) for i in range(blocks)]
+    raw = sum(len(r) for r in recs)
+    n_names = blocks * per_block
+    streams = names.encode_batch_records(recs, coder)
+    out = names.decode_batch_records(streams, coder)
+    assert out == recs
+    comp = sum(len(s) for s in streams)
+    res = {"blocks": blocks, "names": n_names, "record_bytes": raw, "stream_bytes": comp,
+           "ratio": round(raw / comp, 2)}
+    for name, fn in (("encode", lambda: names.encode_batch_records(recs, coder)),
+                     ("decode", lambda: names.decode_batch_records(streams, coder)),
+                     ("decode_and_digest", lambda: names.decode_batch_records(streams, coder, digests=True))):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        t = min(ts)
+        res[name] = {"ms": round(t * 1e3, 2), "mnames_s": round(n_names / t / 1e6, 2),
+                     "record_mbs": round(raw / t / 1e6, 1)}
+    log(f"[names] {n_names} names in {blocks} blocks: encode {res['encode']['mnames_s']} M names/s, "
+        f"decode {res['decode']['mnames_s']} M names/s, +digest {res['decode_and_digest']['mnames_s']} M names/s")
+    return res
+
+
 # ------------------------------------------------------------------------ C5
 def bench_serve(args, coder, dev, state):
     import torch
@@ -577,7 +653,8 @@ def main():
         if "configs" in sections:
             r1, _ = bench_config(coder, dev, oracle, "binned_1mib", [0])
             r2, _ = bench_config(coder, dev, oracle, "normal_64x1mib", [0, 21, 63])
-            configs = {"binned_1mib": r1, "normal_64x1mib": r2, "perm_planes": bench_perm(coder, dev, oracle)}
+            configs = {"binned_1mib": r1, "normal_64x1mib": r2, "perm_planes": bench_perm(coder, dev, oracle),
+                       "names_streams": bench_names(coder)}
         r3, st3 = bench_config(coder, dev, oracle, "markov_64x4mib", [0, 63])
         if "configs" in sections:
             configs["markov_64x4mib"] = r3
@@ -619,6 +696,7 @@ def main():
         "encode": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s", "form": "batched, one persistent launch"},
         "encode_best": c4["encode_best"],
         "crc32c": c4["crc32c"],
+        "xxh3": c4["xxh3"],
         "configs": configs,
         "serving": serving,
     }

@@ -37,8 +37,8 @@ void fqxv_crc_const_fill(void *);
 hipError_t fqxv_launch_crc(const u8 *, const u64 *, const u64 *, const u32 *, const u32 *, u32, const u32 *, u32,
                            const void *, u32 *, u32 *, u32, hipStream_t);
 hipError_t fqxv_launch_perm_join(const u8 *, u64, u32 *, hipStream_t);
-hipError_t fqxv_launch_xxh3(const u8 *, const u64 *, const u64 *, const u32 *, const u32 *, u32, const u32 *, u32,
-                            u64 *, u64 *, hipStream_t);
+hipError_t fqxv_launch_xxh3(const u8 *, const u64 *, const u64 *, const u64 *, u32, u64, u64 *, u64 *,
+                            hipStream_t);
 
 namespace {
 
@@ -590,33 +590,24 @@ int fqxv_rans_crc32c_batch(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_
 static int xxh3_locked(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
                        uint32_t count, uint64_t *d_out, hipStream_t st)
 {
-    // Block items (range, 1 KiB block) of the long inputs, in range order.
-    std::vector<u32> item_range, item_block, first(count);
+    // 1 KiB blocks of the long inputs: range r owns [first[r], first[r + 1]).
+    std::vector<u64> first(count + 1);
+    u64 ni = 0;
     for (u32 r = 0; r < count; r++) {
-        first[r] = (u32)item_range.size();
-        if (len[r] <= 240) continue;
-        const u64 nb = (len[r] - 1) >> 10;
-        if (item_range.size() + nb > 0xFFFFFFFFull) return FQXV_RANS_ERR_INVALID_ARG;
-        for (u64 k = 0; k < nb; k++) {
-            item_range.push_back(r);
-            item_block.push_back((u32)k);
-        }
+        first[r] = ni;
+        if (len[r] > 240) ni += (len[r] - 1) >> 10;
     }
-    const u64 ni = item_range.size();
-    // Device layout: sums[ni][8] off[count] len[count] (u64), first[count]
-    // item_range[ni] item_block[ni] (u32).
-    const u64 need = 64ull * ni + 16ull * count + 4ull * count + 8ull * ni + 64;
+    first[count] = ni;
+    // Device layout: sums[ni][8] off[count] len[count] first[count + 1].
+    const u64 need = 64ull * ni + 24ull * count + 8 + 64;
     int rc = ensure_buf(&c->xxh_buf, &c->xxh_cap, need);
     if (rc) return rc;
-    u64 *d_sums = (u64 *)c->xxh_buf, *d_off = d_sums + 8 * ni, *d_len = d_off + count;
-    u32 *d_first = (u32 *)(d_len + count), *d_ir = d_first + count, *d_ib = d_ir + ni;
+    u64 *d_sums = (u64 *)c->xxh_buf, *d_off = d_sums + 8 * ni, *d_len = d_off + count, *d_first = d_len + count;
     if (hipMemcpyAsync(d_off, off, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(d_len, len, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(d_first, first.data(), 4ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
-        (ni && hipMemcpyAsync(d_ir, item_range.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess) ||
-        (ni && hipMemcpyAsync(d_ib, item_block.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess))
+        hipMemcpyAsync(d_first, first.data(), 8ull * (count + 1), hipMemcpyHostToDevice, st) != hipSuccess)
         return FQXV_RANS_ERR_DEVICE;
-    if (fqxv_launch_xxh3(d_src, d_off, d_len, d_ir, d_ib, (u32)ni, d_first, count, d_sums, d_out, st) != hipSuccess)
+    if (fqxv_launch_xxh3(d_src, d_off, d_len, d_first, count, ni, d_sums, d_out, st) != hipSuccess)
         return FQXV_RANS_ERR_DEVICE;
     // The staging vectors above are host locals: finish before returning.
     return hipStreamSynchronize(st) == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;

@@ -613,8 +613,14 @@ __device__ void decode_job(DecShared &sh, const u8 *src, u8 *dst, const fqxv_ran
         } else if (n != 0) {
             if (order == 0)
                 err = decode_o0(sh, s, end, dst + jb.dst_off, n, jb.dst_cap);
-            else if (order == 1)
+            else if (order == 1) {
+                // Long order-1 blocks are the batch's critical path: their
+                // wave issues first on a shared SIMD.
+                const bool crit = n >= (1u << 20);
+                if (crit) __builtin_amdgcn_s_setprio(3);
                 err = decode_o1(sh, s, end, dst + jb.dst_off, n, jb.dst_cap, ws);
+                if (crit) __builtin_amdgcn_s_setprio(0);
+            }
             else
                 err = FQXV_RANS_ERR_UNKNOWN_ORDER;
         }

@@ -125,34 +125,52 @@ __device__ __forceinline__ u64 stripe_lane(u64 w, u64 w_pair, u64 key)
     return (u64)(u32)k * (k >> 32) + w_pair;
 }
 
-// Thread t: block item t >> 3, lane t & 7.  Eight threads read one block's
-// stripes as 64 contiguous bytes each; lane i takes word i ^ 1 from its
-// neighbour.  The eight lanes of an item are always active together.
+// Eight-lane workers: worker w takes the blocks [w * per, (w + 1) * per)
+// of the concatenation of all ranges' blocks (range r holds blocks
+// [first[r], first[r + 1])), finds its first range by binary search and
+// walks forward.  Lane i reads word i of each stripe (the eight lanes read
+// 64 contiguous bytes) and takes word i ^ 1 from its neighbour; the eight
+// lanes of a worker are always active together.
 __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_blocks(const u8 *__restrict__ src, const u64 *__restrict__ off,
-                                                             const u32 *__restrict__ item_range,
-                                                             const u32 *__restrict__ item_block, u32 nitems,
-                                                             u64 *__restrict__ sums)
+                                                             const u64 *__restrict__ first, u32 count, u64 nitems,
+                                                             u64 per, u64 *__restrict__ sums)
 {
     const u32 i = threadIdx.x & 7;
+    const u64 w = ((u64)blockIdx.x * XXH_THREADS + threadIdx.x) >> 3;
+    u64 it = w * per;
+    const u64 end = it + per < nitems ? it + per : nitems;
+    if (it >= end) return;
     u64 key[16];
 #pragma unroll
     for (u32 s = 0; s < 16; s++) key[s] = sec64(8 * s + 8 * i);
-    const u64 stride = (u64)gridDim.x * XXH_THREADS / 8;
-    for (u64 it = ((u64)blockIdx.x * XXH_THREADS + threadIdx.x) >> 3; it < nitems; it += stride) {
-        const u8 *p = src + off[item_range[it]] + (u64)item_block[it] * XXH_BLOCK;
+    // Largest r with first[r] <= it (first[count] = nitems > it).
+    u32 lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (first[mid] <= it) lo = mid; else hi = mid;
+    }
+    u32 r = lo;
+    u64 r_end = first[r + 1];
+    const u8 *p = src + off[r] + (it - first[r]) * XXH_BLOCK;
+    for (; it < end; it++, p += XXH_BLOCK) {
+        if (it >= r_end) {
+            do r++; while (it >= first[r + 1]);
+            r_end = first[r + 1];
+            p = src + off[r] + (it - first[r]) * XXH_BLOCK;
+        }
         u64 acc = 0;
         if ((((uintptr_t)p) & 7) == 0) {
             const u64 *q = (const u64 *)p + i;
-            u64 w[16];
+            u64 wv[16];
 #pragma unroll
-            for (u32 s = 0; s < 16; s++) w[s] = q[8 * s];
+            for (u32 s = 0; s < 16; s++) wv[s] = q[8 * s];
 #pragma unroll
-            for (u32 s = 0; s < 16; s++) acc += stripe_lane(w[s], (u64)__shfl_xor((long long)w[s], 1, 8), key[s]);
+            for (u32 s = 0; s < 16; s++) acc += stripe_lane(wv[s], (u64)__shfl_xor((long long)wv[s], 1, 8), key[s]);
         } else {
 #pragma unroll 4
             for (u32 s = 0; s < 16; s++) {
-                const u64 w = ld64(p + 64 * s + 8 * i);
-                acc += stripe_lane(w, (u64)__shfl_xor((long long)w, 1, 8), key[s]);
+                const u64 v = ld64(p + 64 * s + 8 * i);
+                acc += stripe_lane(v, (u64)__shfl_xor((long long)v, 1, 8), key[s]);
             }
         }
         sums[it * 8 + i] = acc;
@@ -162,7 +180,7 @@ __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_blocks(const u8 *__restric
 // Eight lanes per range r = t >> 3.
 __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_final(const u8 *__restrict__ src, const u64 *__restrict__ off,
                                                             const u64 *__restrict__ len,
-                                                            const u32 *__restrict__ first_item, u32 count,
+                                                            const u64 *__restrict__ first, u32 count,
                                                             const u64 *__restrict__ sums, u64 *__restrict__ out)
 {
     const u32 i = threadIdx.x & 7;
@@ -179,20 +197,38 @@ __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_final(const u8 *__restrict
             : i == 6 ? P64_5 : P32_1;
     const u64 scr = sec64(128 + 8 * i);
     const u64 nb = (n - 1) / XXH_BLOCK;
-    const u64 *s = sums + (u64)first_item[r] * 8 + i;
+    const u64 *s = sums + first[r] * 8 + i;
     u64 k = 0;
-    // The chain: acc = scramble(acc + sum_k); sums read eight blocks ahead.
-    for (; k + 8 <= nb; k += 8) {
-        u64 v[8];
+    // The chain: acc = scramble(acc + sum_k).  Sums are read in groups of
+    // 16 blocks, the next group's loads issued before the current group's
+    // 16 dependent steps, so a load's latency hides behind a group of steps.
+    constexpr u32 G = 16;
+    if (nb >= 2 * G) {
+        u64 va[G], vb[G];
 #pragma unroll
-        for (u32 j = 0; j < 8; j++) v[j] = s[(k + j) * 8];
+        for (u32 j = 0; j < G; j++) va[j] = s[(k + j) * 8];
+        for (; k + 2 * G <= nb; k += G) {
 #pragma unroll
-        for (u32 j = 0; j < 8; j++) {
-            u64 a = acc + v[j];
+            for (u32 j = 0; j < G; j++) vb[j] = s[(k + G + j) * 8];
+#pragma unroll
+            for (u32 j = 0; j < G; j++) {
+                u64 a = acc + va[j];
+                a ^= a >> 47;
+                a ^= scr;
+                acc = a * P32_1;
+            }
+#pragma unroll
+            for (u32 j = 0; j < G; j++) va[j] = vb[j];
+        }
+        // va holds blocks k .. k + G - 1 (k + G <= nb).
+#pragma unroll
+        for (u32 j = 0; j < G; j++) {
+            u64 a = acc + va[j];
             a ^= a >> 47;
             a ^= scr;
             acc = a * P32_1;
         }
+        k += G;
     }
     for (; k < nb; k++) {
         u64 a = acc + s[k * 8];
@@ -217,18 +253,21 @@ __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_final(const u8 *__restrict
 
 } // namespace
 
-hipError_t fqxv_launch_xxh3(const u8 *src, const u64 *off, const u64 *len, const u32 *item_range,
-                            const u32 *item_block, u32 nitems, const u32 *first_item, u32 count, u64 *sums, u64 *out,
-                            hipStream_t st)
+hipError_t fqxv_launch_xxh3(const u8 *src, const u64 *off, const u64 *len, const u64 *first, u32 count, u64 nitems,
+                            u64 *sums, u64 *out, hipStream_t st)
 {
     if (nitems) {
-        const u64 blocks = ((u64)nitems * 8 + XXH_THREADS - 1) / XXH_THREADS;
-        hipLaunchKernelGGL(k_xxh3_blocks, dim3((u32)(blocks < 65536 ? blocks : 65536)), dim3(XXH_THREADS), 0, st, src,
-                           off, item_range, item_block, nitems, sums);
+        // About 64 Ki workers (2048 workgroups of 32), at least one block each.
+        const u64 workers_max = 2048ull * (XXH_THREADS / 8);
+        const u64 per = (nitems + workers_max - 1) / workers_max;
+        const u64 workers = (nitems + per - 1) / per;
+        const u64 blocks = (workers * 8 + XXH_THREADS - 1) / XXH_THREADS;
+        hipLaunchKernelGGL(k_xxh3_blocks, dim3((u32)blocks), dim3(XXH_THREADS), 0, st, src, off, first, count, nitems,
+                           per, sums);
     }
     if (count) {
         const u64 blocks = ((u64)count * 8 + XXH_THREADS - 1) / XXH_THREADS;
-        hipLaunchKernelGGL(k_xxh3_final, dim3((u32)blocks), dim3(XXH_THREADS), 0, st, src, off, len, first_item, count,
+        hipLaunchKernelGGL(k_xxh3_final, dim3((u32)blocks), dim3(XXH_THREADS), 0, st, src, off, len, first, count,
                            sums, out);
     }
     return hipGetLastError();

@@ -22,7 +22,9 @@ Also reported on the same JSON line:
   critical-path time of one block.
 * ``serving``: the Markov order-1 corpus resident compressed in HBM, Poisson
   arrivals of 1-64 block requests (seed 5) swept from 10% to 120% of the
-  batched peak under a 32 GiB output budget: p50/p99 latency and GB/s.
+  batched peak under a 32 GiB output budget: p50/p99 latency and GB/s, for
+  the persistent engine (one resident grid fed block by block) and the
+  launch-per-admission engine (4 batched launches in flight).
 * ``roofline``: HBM-bound; algorithmic bytes = compressed bytes read +
   uncompressed bytes written, over the decode kernel's measured duration;
   ``traffic`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs.
@@ -521,35 +523,42 @@ def bench_serve(args, coder, dev, state):
     log(f"[serve] batched peak {peak / 1e9:.2f} GB/s ({k} blocks in {t_peak * 1e3:.1f} ms)")
 
     budget = args.serve_budget_gib << 30
-    srv = Server(enc.dst, enc.dst_off, comp_len, corpus.sizes, budget=budget, slots=args.serve_slots,
-                 max_batch=4096)
     mean_req = 32.5 * blk
-    points = []
     bad = []
-    for load in args.serve_loads:
-        rate = load * peak / mean_req
-        arrivals, reqs = poisson_requests(5, args.serve_requests, nb, rate, kmax=64)
+    modes = {}
+    for mode in args.serve_modes:
+        srv = Server(enc.dst, enc.dst_off, comp_len, corpus.sizes, budget=budget, slots=args.serve_slots,
+                     max_batch=4096, mode=mode, waves=args.serve_waves)
+        points = []
+        for load in args.serve_loads:
+            rate = load * peak / mean_req
+            arrivals, reqs = poisson_requests(5, args.serve_requests, nb, rate, kmax=64)
 
-        def verify(i, c, reqs=reqs):
-            if i % 8:
-                return
-            for t, b in zip(srv.output(c, reqs[i]), reqs[i]):
-                if not torch.equal(t, arena[int(corpus.offs[b]):int(corpus.offs[b]) + blk]):
-                    bad.append((load, i, int(b)))
+            def verify(i, c, reqs=reqs, srv=srv, load=load, mode=mode):
+                if i % 8:
+                    return
+                for t, b in zip(srv.output(c, reqs[i]), reqs[i]):
+                    if not torch.equal(t, arena[int(corpus.offs[b]):int(corpus.offs[b]) + blk]):
+                        bad.append((mode, load, i, int(b)))
 
-        recs = run_load(srv, arrivals, reqs, on_done=verify)
-        s = summarize(recs, lambda i, reqs=reqs: len(reqs[i]) * blk)
-        s.update({"offered_load": load, "offered_gbs": round(rate * mean_req / 1e9, 3),
-                  "max_in_flight": srv.max_in_flight, "max_queued": srv.max_queued})
-        points.append(s)
-        log(f"[serve] load {load:.0%}: p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms, {s['gbs']} GB/s, "
-            f"in flight <= {srv.max_in_flight}, queued <= {srv.max_queued}")
-    srv.close()
+            recs = run_load(srv, arrivals, reqs, on_done=verify)
+            s = summarize(recs, lambda i, reqs=reqs: len(reqs[i]) * blk)
+            s.update({"offered_load": load, "offered_gbs": round(rate * mean_req / 1e9, 3),
+                      "max_in_flight": srv.max_in_flight, "max_queued": srv.max_queued})
+            points.append(s)
+            log(f"[serve:{mode}] load {load:.0%}: p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms, {s['gbs']} GB/s, "
+                f"in flight <= {srv.max_in_flight}, queued <= {srv.max_queued}")
+        srv.close()
+        modes[mode] = points
     assert not bad, f"serving outputs differ: {bad[:5]}"
-    assert all(p["errors"] == 0 for p in points)
+    assert all(p["errors"] == 0 for pts in modes.values() for p in pts)
+    first = args.serve_modes[0]
     return {"corpus": "markov_64x4mib, resident compressed in HBM", "batched_peak_gbs": round(peak / 1e9, 3),
             "requests_per_point": args.serve_requests, "request_blocks": "uniform 1-64 distinct ids, seed 5",
-            "budget_gib": args.serve_budget_gib, "slots": args.serve_slots, "points": points,
+            "budget_gib": args.serve_budget_gib, "mode": first, "points": modes[first],
+            "modes": {"persistent": "one resident grid fed block by block from a host-appended ring",
+                      "slots": f"admitted requests packed into batched launches, {args.serve_slots} in flight"},
+            "by_mode": modes,
             "verified": "every 8th request's blocks compared on device with the original bytes"}
 
 
@@ -617,6 +626,9 @@ def main():
     ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.1, 0.25, 0.5, 0.75, 1.0, 1.2])
     ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-waves", type=int, default=0, help="persistent serving grid (0 = occupancy max)")
+    ap.add_argument("--serve-modes", nargs="+", default=["persistent", "slots"],
+                    choices=["persistent", "slots"], help="serving engines to sweep (the first is reported)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB read + 1 GiB write kernel (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()

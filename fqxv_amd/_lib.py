@@ -41,7 +41,7 @@ EXPORTS = [
     "fqxv_rans_perm_split", "fqxv_rans_perm_join", "fqxv_rans_crc32c_batch",
     "fqxv_names_decode_batch", "fqxv_names_status", "fqxv_names_records",
     "fqxv_names_batch_destroy", "fqxv_names_digests", "fqxv_rans_xxh3_batch",
-    "fqxv_names_encode_batch", "fqxv_names_output",
+    "fqxv_names_encode_batch", "fqxv_names_output", "fqxv_rans_server_create_persistent",
 ]
 
 
@@ -112,6 +112,8 @@ def load():
         u32 = ctypes.c_uint32
         L.fqxv_rans_server_create.argtypes = [ctypes.c_int, vp, vp, u32, vp, ctypes.c_uint64, u32, u32,
                                               ctypes.POINTER(vp)]
+        L.fqxv_rans_server_create_persistent.argtypes = [ctypes.c_int, vp, vp, u32, vp, ctypes.c_uint64, u32, u32,
+                                                         u32, ctypes.POINTER(vp)]
         L.fqxv_rans_server_destroy.argtypes = [vp]
         L.fqxv_rans_server_destroy.restype = None
         L.fqxv_rans_server_now.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]

@@ -27,6 +27,7 @@
 //     position its (context, symbol) pair and the state update is
 //     lane-parallel from a (context x symbol) table in LDS.
 #include "rans_device.h"
+#include "rans_serve.h"
 
 namespace {
 
@@ -594,9 +595,9 @@ __device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     return decode_o1_wide(sh, s, end, out, n, K, x, hdr, ws);
 }
 
-// One job: header dispatch (scalar.rs:143-155, lib.rs:226-231).
-__device__ void decode_job(DecShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, i32 *status,
-                           u64 *out_len, u8 *ws)
+// One job: header dispatch (scalar.rs:143-155, lib.rs:226-231).  Returns
+// the status; *n_out is the declared output length.
+__device__ int decode_job_core(DecShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, u8 *ws, u64 &n_out)
 {
     const u8 *s = src + jb.src_off;
     const u8 *end = s + jb.src_len;
@@ -620,11 +621,19 @@ __device__ void decode_job(DecShared &sh, const u8 *src, u8 *dst, const fqxv_ran
                 if (crit) __builtin_amdgcn_s_setprio(3);
                 err = decode_o1(sh, s, end, dst + jb.dst_off, n, jb.dst_cap, ws);
                 if (crit) __builtin_amdgcn_s_setprio(0);
-            }
-            else
+            } else
                 err = FQXV_RANS_ERR_UNKNOWN_ORDER;
         }
     }
+    n_out = n;
+    return err;
+}
+
+__device__ void decode_job(DecShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, i32 *status,
+                           u64 *out_len, u8 *ws)
+{
+    u64 n = 0;
+    const int err = decode_job_core(sh, src, dst, jb, ws, n);
     if (lane_id() == 0) {
         *status = err;
         *out_len = err ? 0 : n;
@@ -660,6 +669,110 @@ __global__ __launch_bounds__(64) void k_rans_decode_one(const u8 *__restrict__ s
     decode_job(sh, src, dst, jb, status, out_len, ws);
 }
 
+// Persistent decode server (queue layout in rans_serve.h).  Workgroup 0 is
+// the dispatcher: it mirrors the host's published count into device memory,
+// and raises the device stop flag when the host stops the server or when the
+// whole server has been idle -- nothing published for `idle_ticks` of the
+// 100 MHz real-time clock and every published job claimed -- then leaves once
+// every decode wave has left.  Decode waves claim a published job
+// (compare-and-swap on head, only below the mirrored tail, so no claim ever
+// waits for a job), decode it exactly as k_rans_decode does, and write the
+// result to the host record with a system-scope release (the decoded bytes
+// are visible before the tag).  They leave on the device stop flag, so the
+// grid leaves as a whole and is relaunched whole by the host when work
+// arrives (a grid that shrank one wave at a time could not grow back while
+// its remaining waves stayed busy).  The dispatcher leaves when no decode
+// wave is left.  Every wait in here ends: the device stop flag is raised at
+// the latest `idle_ticks` after the host stops publishing.  Polls are relaxed: an acquire here
+// would invalidate caches on every poll and evict the tables of the waves
+// that are decoding.  The host's records live in uncached pinned memory and
+// are read after the count that publishes them; the one release is each
+// job's completion tag, which must make the decoded bytes visible.
+__global__ __launch_bounds__(64) void k_rans_serve(const u8 *__restrict__ src, u8 *__restrict__ dst,
+                                                   const fqxv_rans_job *ring_jobs, fqxv_serve_res *ring_res,
+                                                   u32 ring, const fqxv_serve_ctl *ctl, fqxv_serve_dev *dev,
+                                                   u8 *ws, u64 idle_ticks)
+{
+    __shared__ DecShared sh;
+    const u32 l = lane_id();
+    if (blockIdx.x == 0) {
+        u64 last = ~0ull, t_change = __builtin_amdgcn_s_memrealtime();
+        bool stopped = false;
+        for (;;) {
+            u32 go = 1;
+            if (l == 0) {
+                const u64 t = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                const u32 st = __hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                const u64 now = __builtin_amdgcn_s_memrealtime();
+                if (t != last) {
+                    __hip_atomic_store(&dev->dtail, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    last = t;
+                    t_change = now;
+                }
+                if (!stopped &&
+                    (st || (now - t_change > idle_ticks &&
+                            __hip_atomic_load(&dev->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= t))) {
+                    __hip_atomic_store(&dev->dstop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    stopped = true;
+                }
+                if (__hip_atomic_load(&dev->alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) go = 0;
+            }
+            if (uni(__shfl(go, 0)) == 0) return;
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    u8 *myws = ws + (u64)(blockIdx.x - 1) * WS_PER_WAVE;
+    for (;;) {
+        u64 q = 0;
+        u32 act = 0; // 1: job q, 2: leave
+        if (l == 0) {
+            // Idle waves back off (about 3 us doubling to about 100 us between
+            // polls) so that hundreds of them do not crowd one L2 channel.
+            u32 naps = 0;
+            for (;;) {
+                u64 hd = __hip_atomic_load(&dev->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const u64 tl = __hip_atomic_load(&dev->dtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (hd < tl) {
+                    if (__hip_atomic_compare_exchange_strong(&dev->head, &hd, hd + 1, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        q = hd;
+                        act = 1;
+                        break;
+                    }
+                    continue;
+                }
+                if (__hip_atomic_load(&dev->dstop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    act = 2;
+                    break;
+                }
+                for (u32 k = 0; k <= naps; k++) __builtin_amdgcn_s_sleep(127);
+                naps = naps < 31 ? 2 * naps + 1 : 31;
+            }
+            if (act == 2) __hip_atomic_fetch_add(&dev->alive, ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (uni(__shfl(act, 0)) == 2) return;
+        q = (u64)uni(__shfl((u32)q, 0)) | ((u64)uni(__shfl((u32)(q >> 32), 0)) << 32);
+        // The job record, written by the host before it raised the tail.
+        const u64 *jw = (const u64 *)(ring_jobs + (q % ring));
+        u64 w = 0;
+        if (l < 6) w = __hip_atomic_load(jw + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        fqxv_rans_job jb;
+        u64 *jd = (u64 *)&jb;
+#pragma unroll
+        for (u32 k = 0; k < 6; k++)
+            jd[k] = (u64)uni(__shfl((u32)w, k)) | ((u64)uni(__shfl((u32)(w >> 32), k)) << 32);
+        u64 n = 0;
+        const int err = decode_job_core(sh, src, dst, jb, myws, n);
+        if (l == 0) {
+            fqxv_serve_res *r = ring_res + (q % ring);
+            __hip_atomic_store(&r->status, (i32)err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->out_len, err ? 0ull : n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->tag, q + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_reset_counter(u32 *counter) { *counter = 0; }
 
 size_t fqxv_dec_shared_bytes() { return sizeof(DecShared); }
@@ -679,6 +792,15 @@ hipError_t fqxv_launch_decode_one(const u8 *src, u8 *dst, const fqxv_rans_job &j
                                   u64 *out_len, u8 *ws, hipStream_t st)
 {
     hipLaunchKernelGGL(k_rans_decode_one, dim3(1), dim3(64), 0, st, src, dst, jb, status, out_len, ws);
+    return hipGetLastError();
+}
+
+hipError_t fqxv_launch_serve(const u8 *src, u8 *dst, const fqxv_rans_job *ring_jobs, fqxv_serve_res *ring_res,
+                             u32 ring, const fqxv_serve_ctl *ctl, fqxv_serve_dev *dev, u8 *ws, u64 idle_ticks,
+                             u32 decode_waves, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rans_serve, dim3(decode_waves + 1), dim3(64), 0, st, src, dst, ring_jobs, ring_res, ring, ctl,
+                       dev, ws, idle_ticks);
     return hipGetLastError();
 }
 

@@ -9,6 +9,12 @@
 // at once on separate streams, so requests that arrive while others run are
 // admitted as soon as a slot and output memory free up (continuous batching).
 //
+// A second mode (fqxv_rans_server_create_persistent) removes launch
+// boundaries altogether: one persistent grid (k_rans_serve) stays resident and
+// takes blocks from a queue the host appends to, so a request's blocks start
+// as soon as a wave is free, whatever else is running; requests are admitted
+// against the same output budget and complete when their last block does.
+//
 // Output of a request: its blocks in request order, each at a 256-byte aligned
 // offset inside one region of the caller's output pool (block i at out_off + sum over earlier blocks
 // of round_up(n, 256)).  Bytes are those of a plain decode of each block.
@@ -16,6 +22,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <cstdint>
 #include <deque>
 #include <map>
@@ -25,6 +32,11 @@
 #include <vector>
 
 #include "../../include/fqxv_rans_hip.h"
+#include "rans_serve.h"
+
+hipError_t fqxv_launch_serve(const uint8_t *, uint8_t *, const fqxv_rans_job *, fqxv_serve_res *, uint32_t,
+                             const fqxv_serve_ctl *, fqxv_serve_dev *, uint8_t *, uint64_t, uint32_t, hipStream_t);
+hipError_t fqxv_decode_occupancy(int *);
 
 typedef uint8_t u8;
 typedef uint32_t u32;
@@ -113,8 +125,22 @@ struct DeviceGuard {
 
 } // namespace
 
+// Persistent mode: a published job's owner.
+struct JobMeta {
+    u64 ticket;
+    u32 pos;  // block position within the request
+};
+
+// Persistent mode: an admitted request waiting for its blocks.
+struct Active {
+    Req req;
+    u32 remaining = 0;
+    std::vector<i32> status;
+};
+
 struct fqxv_rans_server {
     int device = 0;
+    bool persistent = false;
     const u8 *d_archive = nullptr;
     std::vector<fqxv_rans_job> blocks;
     u8 *d_pool = nullptr;
@@ -126,6 +152,25 @@ struct fqxv_rans_server {
     std::deque<fqxv_rans_done> done;
     std::unordered_map<u64, std::pair<u64, u64>> held; // completed, not yet released
     u64 next_ticket = 1;
+    // Persistent mode (queue layout in rans_serve.h).
+    u32 ring = 0, waves = 0;
+    u64 idle_ticks = 0;
+    fqxv_serve_ctl *h_ctl = nullptr;       // pinned, coherent
+    fqxv_rans_job *h_jobs = nullptr;       // pinned, coherent, `ring` records
+    fqxv_serve_res *h_res = nullptr;       // pinned, coherent, `ring` records
+    fqxv_serve_dev *h_dev = nullptr;       // pinned staging for the device block
+    fqxv_rans_job *dp_jobs = nullptr;      // device views of the three above
+    fqxv_serve_res *dp_res = nullptr;
+    fqxv_serve_ctl *dp_ctl = nullptr;
+    fqxv_serve_dev *d_dev = nullptr;       // device memory
+    u8 *d_ws = nullptr;
+    hipStream_t pstream = nullptr;
+    hipEvent_t pdone = nullptr;
+    bool running = false;
+    u64 pub = 0, oldest = 0, head = 0;     // published, oldest unreaped, claimed at last exit
+    std::vector<JobMeta> meta;             // per ring slot
+    std::vector<char> reaped;              // per ring slot
+    std::unordered_map<u64, Active> active;
     std::chrono::steady_clock::time_point t0;
     std::mutex mu;
 
@@ -230,6 +275,133 @@ int admit(fqxv_rans_server *s)
     return FQXV_RANS_OK;
 }
 
+// Persistent mode: completes finished jobs (tag = index + 1) and requests
+// whose last block finished.
+void reap_p(fqxv_rans_server *s)
+{
+    const double t = s->now();
+    for (u64 q = s->oldest; q < s->pub; q++) {
+        const u32 k = (u32)(q % s->ring);
+        if (s->reaped[k]) continue;
+        const fqxv_serve_res &r = s->h_res[k];
+        if (__atomic_load_n(&r.tag, __ATOMIC_ACQUIRE) != q + 1) continue;
+        s->reaped[k] = 1;
+        auto it = s->active.find(s->meta[k].ticket);
+        Active &a = it->second;
+        a.status[s->meta[k].pos] = __atomic_load_n(&r.status, __ATOMIC_RELAXED);
+        if (--a.remaining == 0) {
+            i32 st = FQXV_RANS_OK;
+            for (i32 v : a.status)
+                if (v != FQXV_RANS_OK) {
+                    st = v;
+                    break;
+                }
+            fqxv_rans_done d;
+            d.ticket = a.req.ticket;
+            d.out_off = a.req.off;
+            d.out_bytes = a.req.bytes;
+            d.t_arrival = a.req.t_arrival;
+            d.t_admit = a.req.t_admit;
+            d.t_done = t;
+            d.status = st;
+            d.nblocks = (u32)a.req.ids.size();
+            s->held[a.req.ticket] = {a.req.off, a.req.bytes};
+            s->done.push_back(d);
+            s->active.erase(it);
+        }
+    }
+    while (s->oldest < s->pub && s->reaped[s->oldest % s->ring]) {
+        s->reaped[s->oldest % s->ring] = 0;
+        s->oldest++;
+    }
+}
+
+// Persistent mode: publishes queued requests (arrival order, head-of-line)
+// while their output fits the budget and their blocks fit the ring; each
+// request's blocks go out largest first.  Then makes sure a grid is running.
+int admit_p(fqxv_rans_server *s)
+{
+    const u64 pub0 = s->pub;
+    const double t = s->now();
+    while (!s->queue.empty()) {
+        Req &r = s->queue.front();
+        const u32 k = (u32)r.ids.size();
+        if (s->pub + k - s->oldest > s->ring) break;
+        u64 off = 0;
+        if (!s->pool->alloc(r.bytes, &off)) break;
+        r.off = off;
+        r.t_admit = t;
+        std::vector<u64> at(k);
+        u64 o = off;
+        for (u32 i = 0; i < k; i++) {
+            at[i] = o;
+            o += round_up(s->blocks[r.ids[i]].dst_cap);
+        }
+        std::vector<u32> ord(k);
+        for (u32 i = 0; i < k; i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](u32 a, u32 b) {
+            return s->blocks[r.ids[a]].dst_cap > s->blocks[r.ids[b]].dst_cap;
+        });
+        for (u32 i : ord) {
+            const u32 slot = (u32)(s->pub % s->ring);
+            fqxv_rans_job j = s->blocks[r.ids[i]];
+            j.dst_off = at[i];
+            s->h_jobs[slot] = j;
+            s->meta[slot] = JobMeta{r.ticket, i};
+            s->pub++;
+        }
+        Active a;
+        a.remaining = k;
+        a.status.assign(k, FQXV_RANS_OK);
+        a.req = std::move(r);
+        s->active.emplace(a.req.ticket, std::move(a));
+        s->queue.pop_front();
+    }
+    // Job records before the count that publishes them.
+    if (s->pub != pub0) __atomic_store_n(&s->h_ctl->tail, s->pub, __ATOMIC_RELEASE);
+    if (s->running) {
+        const hipError_t q = hipEventQuery(s->pdone);
+        if (q == hipErrorNotReady) return FQXV_RANS_OK;
+        if (q != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+        s->running = false;
+        // Every wave has left: the claimed count is final.
+        if (hipMemcpy(&s->head, &s->d_dev->head, sizeof(u64), hipMemcpyDeviceToHost) != hipSuccess)
+            return FQXV_RANS_ERR_DEVICE;
+    }
+    if (s->head >= s->pub) return FQXV_RANS_OK; // nothing unclaimed
+    // (Re)launch: claims continue at `head`; the tail and stop are mirrored
+    // by the dispatcher, seeded here.
+    s->h_dev->head = s->head;
+    s->h_dev->dtail = s->pub;
+    s->h_dev->dstop = 0;
+    s->h_dev->alive = s->waves;
+    if (hipMemcpyAsync(s->d_dev, s->h_dev, sizeof(fqxv_serve_dev), hipMemcpyHostToDevice, s->pstream) != hipSuccess ||
+        fqxv_launch_serve(s->d_archive, s->d_pool, s->dp_jobs, s->dp_res, s->ring, s->dp_ctl, s->d_dev, s->d_ws,
+                          s->idle_ticks, s->waves, s->pstream) != hipSuccess ||
+        hipEventRecord(s->pdone, s->pstream) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    s->running = true;
+    return FQXV_RANS_OK;
+}
+
+void free_persistent(fqxv_rans_server *s)
+{
+    DeviceGuard dg(s->device);
+    if (s->h_ctl) {
+        // The dispatcher sees stop, decode waves finish their jobs and leave.
+        __atomic_store_n(&s->h_ctl->stop, 1u, __ATOMIC_RELEASE);
+        if (s->pstream) (void)hipStreamSynchronize(s->pstream);
+    }
+    if (s->pdone) (void)hipEventDestroy(s->pdone);
+    if (s->pstream) (void)hipStreamDestroy(s->pstream);
+    if (s->d_ws) (void)hipFree(s->d_ws);
+    if (s->d_dev) (void)hipFree(s->d_dev);
+    if (s->h_dev) (void)hipHostFree(s->h_dev);
+    if (s->h_res) (void)hipHostFree(s->h_res);
+    if (s->h_jobs) (void)hipHostFree(s->h_jobs);
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+}
+
 } // namespace
 
 extern "C" {
@@ -237,6 +409,7 @@ extern "C" {
 void fqxv_rans_server_destroy(fqxv_rans_server *s)
 {
     if (!s) return;
+    if (s->persistent) free_persistent(s);
     for (Slot &sl : s->slots) free_slot(s->device, sl);
     delete s->pool;
     delete s;
@@ -288,6 +461,71 @@ int fqxv_rans_server_create(int device, const uint8_t *d_archive, const fqxv_ran
     return FQXV_RANS_OK;
 }
 
+int fqxv_rans_server_create_persistent(int device, const uint8_t *d_archive, const fqxv_rans_job *blocks,
+                                       uint32_t nblocks, uint8_t *d_pool, uint64_t budget, uint32_t ring,
+                                       uint32_t waves, uint32_t idle_ms, fqxv_rans_server **out)
+{
+    if (!out || !d_archive || !blocks || nblocks == 0 || !d_pool || budget < ALIGN || ring == 0 || idle_ms == 0 ||
+        idle_ms > 60000)
+        return FQXV_RANS_ERR_INVALID_ARG;
+    *out = nullptr;
+    auto *s = new (std::nothrow) fqxv_rans_server();
+    if (!s) return FQXV_RANS_ERR_DEVICE;
+    s->device = device;
+    s->persistent = true;
+    s->d_archive = d_archive;
+    s->blocks.assign(blocks, blocks + nblocks);
+    for (const fqxv_rans_job &j : s->blocks)
+        if (j.dst_cap > budget) {
+            delete s;
+            return FQXV_RANS_ERR_INVALID_ARG;
+        }
+    s->budget = budget & ~(ALIGN - 1);
+    s->max_batch = ring; // a request's blocks must fit the ring at once
+    s->ring = ring;
+    s->idle_ticks = (u64)idle_ms * 100000; // 100 MHz real-time clock
+    s->d_pool = d_pool;
+    s->pool = new (std::nothrow) Pool(s->budget);
+    s->t0 = std::chrono::steady_clock::now();
+    s->meta.resize(ring);
+    s->reaped.assign(ring, 0);
+    DeviceGuard dg(device);
+    hipDeviceProp_t prop;
+    int occ = 0;
+    bool ok = s->pool != nullptr && hipGetDeviceProperties(&prop, device) == hipSuccess &&
+              fqxv_decode_occupancy(&occ) == hipSuccess;
+    if (ok) {
+        // A resident grid that filled every wave slot would starve the
+        // kernels that consume its output (they could not be scheduled until
+        // the server went idle): by default it takes half of the chip.
+        const u32 slots_all = (u32)std::max(1, occ) * (u32)prop.multiProcessorCount;
+        const u32 maxw = slots_all - 1; // one slot for the dispatcher
+        s->waves = waves ? std::min(waves, maxw) : std::max(1u, slots_all / 2 - 1);
+    }
+    const unsigned pin = hipHostMallocMapped | hipHostMallocCoherent;
+    ok = ok && hipHostMalloc((void **)&s->h_ctl, sizeof(fqxv_serve_ctl), pin) == hipSuccess &&
+         hipHostMalloc((void **)&s->h_jobs, (size_t)ring * sizeof(fqxv_rans_job), pin) == hipSuccess &&
+         hipHostMalloc((void **)&s->h_res, (size_t)ring * sizeof(fqxv_serve_res), pin) == hipSuccess &&
+         hipHostMalloc((void **)&s->h_dev, sizeof(fqxv_serve_dev), 0) == hipSuccess &&
+         hipHostGetDevicePointer((void **)&s->dp_ctl, s->h_ctl, 0) == hipSuccess &&
+         hipHostGetDevicePointer((void **)&s->dp_jobs, s->h_jobs, 0) == hipSuccess &&
+         hipHostGetDevicePointer((void **)&s->dp_res, s->h_res, 0) == hipSuccess &&
+         hipMalloc((void **)&s->d_dev, sizeof(fqxv_serve_dev)) == hipSuccess &&
+         hipMalloc((void **)&s->d_ws, (size_t)s->waves * (256ull * 4096 * 2 + 256ull * 256 * 4)) == hipSuccess &&
+         hipStreamCreateWithFlags(&s->pstream, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&s->pdone, hipEventDisableTiming) == hipSuccess;
+    if (ok) {
+        memset(s->h_ctl, 0, sizeof(fqxv_serve_ctl));
+        memset(s->h_res, 0, (size_t)ring * sizeof(fqxv_serve_res));
+    }
+    if (!ok) {
+        fqxv_rans_server_destroy(s);
+        return FQXV_RANS_ERR_DEVICE;
+    }
+    *out = s;
+    return FQXV_RANS_OK;
+}
+
 int fqxv_rans_server_now(fqxv_rans_server *s, double *t)
 {
     if (!s || !t) return FQXV_RANS_ERR_INVALID_ARG;
@@ -320,11 +558,17 @@ int fqxv_rans_server_poll(fqxv_rans_server *s, uint32_t *queued, uint32_t *in_fl
     if (!s) return FQXV_RANS_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
     DeviceGuard dg(s->device);
-    int rc = reap(s);
-    if (!rc) rc = admit(s);
+    int rc;
+    if (s->persistent) {
+        reap_p(s);
+        rc = admit_p(s);
+    } else {
+        rc = reap(s);
+        if (!rc) rc = admit(s);
+    }
     if (queued) *queued = (u32)s->queue.size();
     if (in_flight) {
-        u32 n = 0;
+        u32 n = (u32)s->active.size();
         for (const Slot &sl : s->slots) n += (u32)sl.reqs.size();
         *in_flight = n;
     }

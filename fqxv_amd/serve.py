@@ -41,10 +41,14 @@ class Server:
     ``archive``: torch uint8 device tensor of compressed streams;
     ``src_off``/``src_len``: each block's stream; ``n``: each block's decoded
     length.  ``budget``: bytes of decoded output that may be held at once.
+    ``mode``: ``"slots"`` (admitted requests packed into batched launches, up
+    to ``slots`` at once) or ``"persistent"`` (one resident grid of ``waves``
+    decode waves, 0 = half of the chip, fed block by block through a ``ring``
+    of jobs; the grid leaves after ``idle_ms`` without new work).
     """
 
     def __init__(self, archive, src_off, src_len, n, budget: int, slots: int = 4, max_batch: int = 4096,
-                 device: int = 0):
+                 device: int = 0, mode: str = "slots", ring: int = 16384, waves: int = 0, idle_ms: int = 200):
         import torch
         self._L = _lib.load()
         self.archive = archive
@@ -55,10 +59,18 @@ class Server:
         self.budget = int(budget)
         self.pool = torch.empty(self.budget, dtype=torch.uint8, device=archive.device)
         h = ctypes.c_void_p()
-        rc = self._L.fqxv_rans_server_create(device, ctypes.c_void_p(archive.data_ptr()),
-                                             ctypes.c_void_p(self._jobs.ctypes.data), len(self.n),
-                                             ctypes.c_void_p(self.pool.data_ptr()), self.budget,
-                                             slots, max_batch, ctypes.byref(h))
+        self.mode = mode
+        if mode == "persistent":
+            rc = self._L.fqxv_rans_server_create_persistent(
+                device, ctypes.c_void_p(archive.data_ptr()), ctypes.c_void_p(self._jobs.ctypes.data), len(self.n),
+                ctypes.c_void_p(self.pool.data_ptr()), self.budget, ring, waves, idle_ms, ctypes.byref(h))
+        elif mode == "slots":
+            rc = self._L.fqxv_rans_server_create(device, ctypes.c_void_p(archive.data_ptr()),
+                                                 ctypes.c_void_p(self._jobs.ctypes.data), len(self.n),
+                                                 ctypes.c_void_p(self.pool.data_ptr()), self.budget,
+                                                 slots, max_batch, ctypes.byref(h))
+        else:
+            raise ValueError(f"unknown serving mode {mode!r}")
         _raise(rc)
         self._h = h
         self._buf = (_lib.Done * 256)()

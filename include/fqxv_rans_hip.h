@@ -212,6 +212,20 @@ typedef struct fqxv_rans_done {
 int fqxv_rans_server_create(int device, const uint8_t *d_archive, const fqxv_rans_job *blocks, uint32_t nblocks,
                             uint8_t *d_pool, uint64_t budget, uint32_t slots, uint32_t max_batch,
                             fqxv_rans_server **out);
+/* The same service without launch boundaries: one persistent grid of
+ * `waves` decode waves (0 = half of the chip's wave slots, which leaves room
+ * for the kernels that consume the output; at most all slots but one) stays
+ * resident and takes
+ * blocks from a ring of `ring` jobs that the host appends to (pinned host
+ * memory; a dispatcher wave mirrors the published count into device memory),
+ * so an admitted request's blocks start as soon as a wave is free, whatever
+ * else is running.  A request may name at most `ring` blocks.  The grid
+ * leaves as a whole once nothing has been published for `idle_ms`
+ * (1..60000) and every job is claimed, and is relaunched when work arrives;
+ * destroy stops it.  Same submit/poll/take/release calls. */
+int fqxv_rans_server_create_persistent(int device, const uint8_t *d_archive, const fqxv_rans_job *blocks,
+                                       uint32_t nblocks, uint8_t *d_pool, uint64_t budget, uint32_t ring,
+                                       uint32_t waves, uint32_t idle_ms, fqxv_rans_server **out);
 void fqxv_rans_server_destroy(fqxv_rans_server *srv);
 int fqxv_rans_server_now(fqxv_rans_server *srv, double *t);
 /* Queue a request for blocks ids[0..k-1]; t_arrival < 0 means now. */

@@ -34,7 +34,7 @@ def _archive(coder, oracle):
     return blocks, enc, ol
 
 
-def _server(enc, ol, blocks, budget, slots=3, extra_bad=False):
+def _server(enc, ol, blocks, budget, slots=3, extra_bad=False, mode="slots"):
     from fqxv_amd.serve import Server
     src_off = list(enc.dst_off)
     src_len = list(ol)
@@ -43,10 +43,15 @@ def _server(enc, ol, blocks, budget, slots=3, extra_bad=False):
         src_off.append(enc.dst_off[0])
         src_len.append(5)
         n.append(len(blocks[0]))
-    return Server(enc.dst, src_off, src_len, n, budget=budget, slots=slots, max_batch=256)
+    return Server(enc.dst, src_off, src_len, n, budget=budget, slots=slots, max_batch=256, mode=mode,
+                  ring=256, idle_ms=50)
 
 
-def test_serving_alone_equals_in_mix(coder, oracle):
+MODES = ["slots", "persistent"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_serving_alone_equals_in_mix(coder, oracle, mode):
     from fqxv_amd.serve import poisson_requests, run_load
     blocks, enc, ol = _archive(coder, oracle)
     nb = len(blocks)
@@ -54,7 +59,7 @@ def test_serving_alone_equals_in_mix(coder, oracle):
     sample = [0, 7, 13, 29, 44, 59]
 
     alone = {}
-    srv = _server(enc, ol, blocks, budget=16 << 20)
+    srv = _server(enc, ol, blocks, budget=16 << 20, mode=mode)
     for i in sample:
         def grab(_, c, ids=reqs[i]):
             alone[i] = [t.cpu().numpy().tobytes() for t in srv.output(c, ids)]
@@ -63,7 +68,7 @@ def test_serving_alone_equals_in_mix(coder, oracle):
     srv.close()
 
     seen = {}
-    srv = _server(enc, ol, blocks, budget=6 << 20)  # small budget: admission must wait
+    srv = _server(enc, ol, blocks, budget=6 << 20, mode=mode)  # small budget: admission must wait
 
     def check(i, c):
         outs = srv.output(c, reqs[i])
@@ -80,12 +85,13 @@ def test_serving_alone_equals_in_mix(coder, oracle):
     srv.close()
 
 
-def test_serving_errors_stay_per_request(coder, oracle):
+@pytest.mark.parametrize("mode", MODES)
+def test_serving_errors_stay_per_request(coder, oracle, mode):
     import fqxv_amd
     from fqxv_amd.serve import run_load
     blocks, enc, ol = _archive(coder, oracle)
     nb = len(blocks)
-    srv = _server(enc, ol, blocks, budget=32 << 20, extra_bad=True)
+    srv = _server(enc, ol, blocks, budget=32 << 20, extra_bad=True, mode=mode)
     reqs = [np.array([1, 2, 3], np.uint32), np.array([4, nb], np.uint32), np.array([5, 6], np.uint32)]
     outs = {}
 
@@ -106,9 +112,32 @@ def test_serving_errors_stay_per_request(coder, oracle):
     sizes = [len(b) for b in blocks]
     big = int(np.argmax(sizes))
     with pytest.raises(ValueError):  # a block larger than the whole budget
-        _server(enc, ol, blocks, budget=1024)
-    small = _server(enc, ol, blocks, budget=(sizes[big] + 255) // 256 * 256)
+        _server(enc, ol, blocks, budget=1024, mode=mode)
+    small = _server(enc, ol, blocks, budget=(sizes[big] + 255) // 256 * 256, mode=mode)
     second = int(np.argsort(sizes)[-2])
     with pytest.raises(ValueError):  # a request larger than the whole budget
         small.submit([big, second])
     small.close()
+
+
+def test_persistent_grid_leaves_when_idle_and_comes_back(coder, oracle):
+    """The resident grid drains after idle_ms without work; later requests
+    relaunch it; destroy returns (every wave has an exit)."""
+    import time
+    from fqxv_amd.serve import run_load
+    blocks, enc, ol = _archive(coder, oracle)
+    srv = _server(enc, ol, blocks, budget=32 << 20, mode="persistent")
+    for rnd in range(3):
+        ids = np.array([rnd, 10 + rnd, 20 + rnd, 47], np.uint32)
+        got = {}
+
+        def grab(_, c, ids=ids):
+            got[0] = [t.cpu().numpy().tobytes() for t in srv.output(c, ids)]
+
+        recs = run_load(srv, [0.0], [ids], on_done=grab)
+        assert recs[0].status == 0
+        assert got[0] == [blocks[b] for b in ids]
+        time.sleep(0.2)  # > idle_ms: the grid leaves
+    t0 = time.perf_counter()
+    srv.close()
+    assert time.perf_counter() - t0 < 5.0

@@ -224,6 +224,10 @@ def bench_c4(args, coder, dev, oracle):
         torch.cuda.synchronize()
         assert torch.equal(out[:n_total], arena[:n_total])
     xxh = bench_xxh3(coder, arena, corpus.offs, corpus.sizes, args.workload) if args.crc else None
+    # Fingerprint of the generated corpus (one XXH3-64 over all of it, on the
+    # device), so a result can be tied to the exact input bytes.
+    fp = "%016x" % (int(coder.xxh3_batch(arena, [0], [n_total]).cpu().numpy().view(np.uint64)[0]))
+    pcie = bench_pcie(coder, enc, comp_len, corpus, arena) if args.pcie else None
 
     # CPU-baseline sample (host copies of a subset of the streams).
     cpu_idx = np.arange(0, nb, args.cpu_every)
@@ -231,6 +235,8 @@ def bench_c4(args, coder, dev, oracle):
     res = {
         "crc32c": crc,
         "xxh3": xxh,
+        "fingerprint": fp,
+        "pcie": pcie,
         "encode_best": best,
         "corpus": corpus, "value": value, "t_step": t_step, "t_kern": t_kern, "n_total": n_total,
         "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "perblock": pb,
@@ -412,6 +418,52 @@ def bench_crc(coder, oracle, enc, comp_len, corpus_name):
     return {"corpus": f"compressed blocks of {corpus_name}", "checked_vs_oracle": len(idx),
             "timing": "host wall clock per call (the call includes its host-side planning and returns "
                       "when the checksums are written)", **res}
+
+
+def bench_pcie(coder, enc, comp_len, corpus, arena):
+    """The same decode from host buffers: the compressed streams in pinned
+    host memory are copied to HBM, decoded in one batched launch, and the
+    output copied back to pinned host memory, each phase timed with events
+    on one stream.  Not the headline (inputs there are resident in HBM)."""
+    import torch
+    n_total = corpus.total
+    span = int(enc.dst_off[-1] + comp_len[-1])
+    h_in = torch.empty(span, dtype=torch.uint8, pin_memory=True)
+    h_in.copy_(enc.dst[:span])
+    h_out = torch.empty(n_total, dtype=torch.uint8, pin_memory=True)
+    d_in = torch.empty(span + 64, dtype=torch.uint8, device=arena.device)
+    stream = torch.cuda.current_stream()
+    best = None
+    for _ in range(2):
+        e = events(3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e[0][0].record(stream)
+        d_in[:span].copy_(h_in, non_blocking=True)
+        e[0][1].record(stream)
+        e[1][0].record(stream)
+        d = coder.decode_batch(d_in, enc.dst_off, comp_len, corpus.sizes, stream=stream)
+        e[1][1].record(stream)
+        e[2][0].record(stream)
+        h_out.copy_(d.dst[:n_total], non_blocking=True)
+        e[2][1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = [a.elapsed_time(b) for a, b in e]
+        if best is None or wall < best[0]:
+            best = (wall, ms)
+        del d
+    k = 1 << 20
+    assert torch.equal(h_out[:k], arena[:k].cpu()) and torch.equal(h_out[-k:], arena[n_total - k:n_total].cpu())
+    wall, ms = best
+    res = {"gbs": round(n_total / wall / 1e9, 2), "ms": round(wall * 1e3, 1), "h2d_ms": round(ms[0], 1),
+           "decode_ms": round(ms[1], 1), "d2h_ms": round(ms[2], 1),
+           "h2d_gbs": round(span / ms[0] / 1e6, 1), "d2h_gbs": round(n_total / ms[2] / 1e6, 1),
+           "form": "pinned host -> HBM, batched decode (eager, LPT order), HBM -> pinned host, one stream"}
+    log(f"[pcie] host-to-host decode {res['gbs']} GB/s ({res['h2d_ms']} + {res['decode_ms']} + {res['d2h_ms']} ms)")
+    del h_in, h_out, d_in
+    torch.cuda.empty_cache()
+    return res
 
 
 def bench_xxh3(coder, arena, offs, lens, corpus_name):
@@ -618,6 +670,7 @@ def main():
     ap.add_argument("--no-perblock", action="store_true")
     ap.add_argument("--encode-best", action=argparse.BooleanOptionalAction, default=True)
     ap.add_argument("--crc", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--pcie", action=argparse.BooleanOptionalAction, default=True)
     ap.add_argument("--cpu-every", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -696,7 +749,9 @@ def main():
                    "blocks": c4["nb"], "uncompressed_bytes": n_total, "compressed_bytes": c_total,
                    "order1_blocks": int(corpus.orders.sum()),
                    "launch": "hipGraph replay of one persistent-kernel launch over all blocks",
-                   "grid": tuned.grid, "queue_policy": tuned.policy},
+                   "grid": tuned.grid, "queue_policy": tuned.policy,
+                   "generator": "counter-based splitmix64 (fqxv_amd/csrc/synth.c), seeds 1-5",
+                   "corpus_xxh3": c4["fingerprint"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_rans_decode", "kernel_ms": round(t_kern * 1e3, 3)},
@@ -709,6 +764,7 @@ def main():
         "encode_best": c4["encode_best"],
         "crc32c": c4["crc32c"],
         "xxh3": c4["xxh3"],
+        "pcie_inclusive": c4["pcie"],
         "configs": configs,
         "serving": serving,
     }

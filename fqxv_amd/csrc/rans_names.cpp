@@ -473,8 +473,10 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
         hi = std::max<u64>(hi, off[i] + len[i]);
     }
     if (count == 0) lo = 0;
-    // Jobs: every sub-stream of >= 9 bytes within its static bound (the
-    // flat streams' max_out; a plane's n_delta never exceeds n_records).
+    // Jobs: every sub-stream of >= 9 bytes within its static bound: the flat
+    // streams' max_out; a plane's n_delta never exceeds n_records, and planes
+    // are only read once n_records <= counts length <= max_out has been
+    // checked (lib.rs:565-567), so min(n_records, max_out) bounds them.
     std::vector<fqxv_rans_job> jobs;
     std::vector<u64> job_off;
     u64 dst_total = 0;
@@ -483,7 +485,7 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
         for (Event &ev : S.ev) {
             if (ev.kind == EV_WIDTH) plane_part = true;
             if (ev.kind != EV_SUB) continue;
-            const u64 bound = plane_part ? S.n_records : S.max_out;
+            const u64 bound = plane_part ? std::min(S.n_records, S.max_out) : S.max_out;
             if (ev.len < 9) {
                 ev.host_st = FQXV_RANS_ERR_TRUNCATED; // scalar.rs: tag + u64 n need 9 bytes
                 continue;

@@ -141,3 +141,27 @@ def test_persistent_grid_leaves_when_idle_and_comes_back(coder, oracle):
     t0 = time.perf_counter()
     srv.close()
     assert time.perf_counter() - t0 < 5.0
+
+
+def test_persistent_ring_backpressure(coder, oracle):
+    """A ring far smaller than the offered work: jobs wait for ring slots,
+    slots are reused many times, every output is still the original."""
+    from fqxv_amd.serve import Server, run_load
+    blocks, enc, ol = _archive(coder, oracle)
+    nb = len(blocks)
+    srv = Server(enc.dst, list(enc.dst_off), list(ol), [len(b) for b in blocks], budget=64 << 20,
+                 mode="persistent", ring=16, idle_ms=50)
+    rng = np.random.default_rng(8)
+    reqs = [np.sort(rng.choice(nb, int(rng.integers(1, 9)), replace=False)).astype(np.uint32) for _ in range(40)]
+    seen = {}
+
+    def grab(i, c):
+        seen[i] = [t.cpu().numpy().tobytes() for t in srv.output(c, reqs[i])]
+
+    recs = run_load(srv, [0.0] * len(reqs), reqs, on_done=grab)
+    for i, r in enumerate(recs):
+        assert r.status == 0, i
+        assert seen[i] == [blocks[b] for b in reqs[i]], i
+    with pytest.raises(ValueError):  # more blocks than the ring holds
+        srv.submit(np.arange(17, dtype=np.uint32))
+    srv.close()

@@ -25,7 +25,8 @@
 //     chain is then 32 v_readlane steps through those candidate vectors, the
 //     code being the lane of the next context.  v_writelane hands each
 //     position its (context, symbol) pair and the state update is
-//     lane-parallel from a (context x symbol) table in LDS.
+//     lane-parallel from a (context x symbol) table in LDS.  More than 64
+//     contexts spread each candidate vector over up to four VGPRs.
 #include "rans_device.h"
 #include "rans_serve.h"
 
@@ -496,62 +497,118 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     return 0;
 }
 
-// General path (K > 64 or a wide symbol set): serial chain through global
-// tables.  Correct but slow; no synthetic or fixture stream reaches it.
-// Workspace: sym16[slot * 256 + d] (u16 symbol), fcw[d * 256 + sym].
-__device__ int decode_o1_wide(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
-                              u64 hdr, u8 *ws)
+// Wide path: more than 64 contexts, or 64 contexts and one more symbol
+// (|contexts u symbols| <= 256).  The same column gather and v_readlane chain
+// as the fast path, with four contexts packed per lane: byte q of lane l of
+// column j is the code of the symbol context 64q + l decodes at slot_j, so a
+// chain step is one v_readlane at lane (c & 63) (the lane select keeps the low
+// six bits) and a scalar extract of byte c >> 6.  Bytes of contexts >= K are
+// never chosen: a code >= K leaves the fast round and the exact path stops on
+// it.  The (context x symbol) entry table lives in the workspace:
+// t32[slot * 64 + (d & 63)] byte (d >> 6) (1 MiB), and
+// fcw[d * 256 + code] = (f-1) << 20 | cum << 8 | sym.
+__device__ int decode_o1_multi(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x, u64 hdr,
+                               u8 *ws)
 {
     const u32 l = lane_id();
-    u16 *sym16 = (u16 *)ws;
+    u8 *t8 = ws;
     u32 *fcw = (u32 *)(ws + WS_CODES);
     const u8 *t = s + HDR0 + BITMAP;
     for (u32 d = 0; d < K; d++, t += 512) {
-        (void)load_freq_table(sh, t, end);
-        for (int i = 0; i < 4; i++) {
-            const u32 e = sh.fc[4 * l + i];
-            fcw[d * 256 + 4 * l + i] = ((e >> 16) << 12) | (e & 0xffff);
-        }
-        for_each_slot(sh, [&](u32 k, u32 sym, u32, u32) { sym16[(u64)k * 256 + d] = (u16)sym; });
+        (void)load_freq_table(sh, t, end); // validated already
+        const u32 col = (d & 63) * 4 + (d >> 6);
+        for_each_slot(sh, [&](u32 k, u32 sym, u32 f, u32 cum) {
+            const u32 ul = sh.ulane[sym];
+            t8[k * 256 + col] = (u8)ul;
+            if (k == cum) fcw[d * 256 + ul] = ((f - 1) << 20) | (cum << 8) | sym;
+        });
         __syncthreads();
     }
     __threadfence();
+    const __amdgpu_buffer_rsrc_t t8rs = __builtin_amdgcn_make_buffer_rsrc(t8, (short)0, (int)(TOTFREQ * 256), 0x00020000);
 
     WordRing wr;
     wr.init(s + hdr, end);
     const u64 avail = ((u64)(end - s) - hdr) / 2;
     wr.reset(sh.ring, 0);
 
-    u32 prev = sh.dense[0];
+    u32 prev = sh.dense[0]; // context of symbol 0 is 0 (scalar.rs:203)
+    const bool lo = l < NSTATES;
     const u64 rounds = (n + NSTATES - 1) / NSTATES;
+    const u64 full = n / NSTATES;
     u64 pos = 0;
     u32 row = 0;
     u8 *o = out;
-    for (u64 r = 0; r < rounds; r++) {
-        const u32 cntr = (r + 1) * NSTATES <= n ? NSTATES : (u32)(n - r * NSTATES);
-        u32 symv = 0, ctxv = 0;
-        u32 bad = cntr;
-        for (u32 j = 0; j < cntr; j++) {
-            if (prev == 0xffff) {
-                bad = j;
-                break;
+    u64 r = 0;
+    // One chain step: the code context c decodes to, from its packed column.
+    auto step = [](u32 cv, u32 c) -> u32 { return (__builtin_amdgcn_readlane(cv, c) >> ((c >> 3) & 24)) & 0xffu; };
+    while (r < rounds) {
+        const u32 slot = x & (TOTFREQ - 1);
+        const u32 rowoff = slot * 256;
+        u32 cand[NSTATES];
+#pragma unroll
+        for (u32 j = 0; j < NSTATES; j++)
+            cand[j] = __builtin_amdgcn_raw_buffer_load_b32(t8rs, l * 4, __builtin_amdgcn_readlane(rowoff, j), 0);
+        if (r < full && prev < K && pos + NSTATES <= avail) {
+            // Fast round (see decode_o1_fast).
+            const u32 x0 = x;
+            const u64 pos0 = pos;
+            u32 c = prev;
+            u32 sv = 0;
+#pragma unroll
+            for (u32 j = 0; j < NSTATES; j++) {
+                c = step(cand[j], c);
+                sv = writelane(c, j, sv);
             }
-            const u32 sj = __builtin_amdgcn_readlane(x, j) & (TOTFREQ - 1);
-            const u32 sym = uni(sym16[(u64)sj * 256 + prev]) & 0xff;
-            if (l == j) {
-                symv = sym;
-                ctxv = prev;
+            // Position j's context is position j-1's symbol.
+            u32 ctxv = (u32)__builtin_amdgcn_update_dpp((int)sv, (int)sv, 0x111, 0xf, 0xf, false);
+            ctxv = writelane(prev, 0, ctxv);
+            ctxv = writelane(__builtin_amdgcn_readlane(sv, 15), 16, ctxv);
+            const u32 ent = lo ? fcw[(ctxv & 0xff) * 256 + (sv & 0xff)] : 0u;
+            const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
+            const bool need = lo && v < RANS_L;
+            const u64 m = ballot(need);
+            const u32 w = ((const u16 *)sh.ring)[(u32)(pos % RING_WORDS) + __builtin_amdgcn_mbcnt_lo((u32)m, 0u)];
+            if (lo) x = need ? ((v << 16) | w) : v;
+            pos += (u32)__popcll(m);
+            if (lo) stage_byte(sh, row, ent);
+            if (ballot(lo && (x < RANS_L || (ctxv & 0xff) >= K)) == 0) {
+                prev = c;
+                r++;
+                wr.advance(sh.ring, pos);
+                if (++row == OUT_ROUNDS) {
+                    flush(sh, o, OUT_BYTES);
+                    o += OUT_BYTES;
+                    row = 0;
+                }
+                continue;
             }
-            prev = sh.dense[sym];
+            x = x0;
+            pos = pos0;
         }
-        const bool act = l < bad;
-        const u32 fc = act ? fcw[ctxv * 256 + symv] : (1u << 12);
-        const u32 f = fc >> 12, cum = fc & 0xfff;
-        const u32 v = f * (x >> SCALE_BITS) + (x & (TOTFREQ - 1)) - cum;
+        // Exact path: absent contexts, truncation, multi-word states, tail.
+        if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        const u32 cntr = r < full ? NSTATES : (u32)(n - r * NSTATES);
+        u32 c = prev;
+        u32 cv = 0;
+#pragma unroll
+        for (u32 j = 0; j < NSTATES; j++) {
+            const u32 nxt = c < K ? step(cand[j], c) : 0xffu;
+            cv = writelane(c | (nxt << 8), j, cv);
+            c = nxt;
+        }
+        prev = c;
+        // First position whose context is no context: that symbol fails.
+        const u64 nb = ballot(lo && l < cntr && (cv & 0xff) >= K);
+        const u32 jb = nb ? (u32)__builtin_ctzll(nb) : NSTATES;
+        const bool act = lo && l < cntr && l < jb;
+        const u32 ent = act ? fcw[(cv & 0xff) * 256 + ((cv >> 8) & 0xff)] : 0u;
+        const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
         int err = renorm(sh, wr, x, v, act, pos, avail);
         if (err) return err;
-        if (act) stage_byte(sh, row, symv);
-        if (bad < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        if (act) stage_byte(sh, row, ent);
+        if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        r++;
         if (++row == OUT_ROUNDS) {
             // The last round may be partial: never write past out + n.
             const u64 left = n - (u64)(o - out);
@@ -592,7 +649,7 @@ __device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     }
     __syncthreads();
     if (K <= UMAX && nu <= UMAX) return decode_o1_fast(sh, s, end, out, n, K, x, hdr, ws);
-    return decode_o1_wide(sh, s, end, out, n, K, x, hdr, ws);
+    return decode_o1_multi(sh, s, end, out, n, K, x, hdr, ws);
 }
 
 // One job: header dispatch (scalar.rs:143-155, lib.rs:226-231).  Returns

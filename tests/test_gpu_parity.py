@@ -249,3 +249,54 @@ def test_encode_best_batch_matches_oracle(coder, oracle):
     out = d.dst.cpu().numpy()
     for j, b in enumerate(blocks):
         assert out[int(d.dst_off[j]):int(d.dst_off[j]) + len(b)].tobytes() == b, j
+
+
+@pytest.mark.parametrize("k", [64, 65, 100, 128, 129, 192, 193, 255, 256])
+def test_order1_wide_alphabets(coder, oracle, k):
+    """Order-1 with more than 64 contexts (each candidate vector spread over
+    up to four VGPRs): device encode == oracle, device decode == input, for
+    K on both sides of every 64-context boundary."""
+    import torch
+    rng = np.random.default_rng(900 + k)
+    n = 40000 + k
+    data = rng.integers(0, k, n).astype(np.uint8).tobytes()
+    if k == 64:
+        data = data + bytes([200])  # 64 contexts and one symbol that is no context
+    ref = oracle.encode(data, 1)
+    assert coder.encode(data, 1) == ref
+    assert coder.decode(ref) == data
+    # Batched, next to a short one of the same width.
+    short = rng.integers(0, k, 777).astype(np.uint8).tobytes()
+    blobs = [ref, oracle.encode(short, 1)]
+    offs = np.array([0, len(ref)], np.uint64)
+    lens = np.array([len(b) for b in blobs], np.uint64)
+    arena = torch.from_numpy(np.frombuffer(b"".join(blobs) + bytes(64), np.uint8).copy()).cuda()
+    r = coder.decode_batch(arena, offs, lens, [len(data), len(short)])
+    torch.cuda.synchronize()
+    assert list(r.status.cpu().numpy()) == [0, 0]
+    out = r.dst.cpu().numpy()
+    assert out[:len(data)].tobytes() == data
+    assert out[int(r.dst_off[1]):int(r.dst_off[1]) + len(short)].tobytes() == short
+
+
+def test_order1_wide_malformed_same_error(coder, oracle):
+    rng = np.random.default_rng(77)
+    data = rng.integers(0, 180, 20000).astype(np.uint8).tobytes()
+    ref = bytearray(oracle.encode(data, 1))
+    cases = [bytes(ref[:len(ref) - 3]), bytes(ref[:len(ref) // 2])]
+    for _ in range(40):
+        m = bytearray(ref)
+        i = int(rng.integers(0, len(m)))
+        m[i] ^= int(rng.integers(1, 256))
+        cases.append(bytes(m))
+    checked = 0
+    for c in cases:
+        n = oracle.peek_len(c)
+        if n is not None and n > len(data) * 4:  # a mutated length: skip the huge allocation
+            continue
+        rc_o, out_o, _ = oracle.decode_ex(c)
+        assert _err_code(lambda: coder.decode(c)) == rc_o
+        if rc_o == 0:
+            assert coder.decode(c) == out_o
+        checked += 1
+    assert checked > 30

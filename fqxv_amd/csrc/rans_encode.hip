@@ -6,8 +6,9 @@
 //
 // One workgroup = one wave64 encodes one block at a time (persistent grid,
 // device work queue).  Per block:
-//   1. histogram with LDS atomics (order-1: dense-remapped pair counts in
-//      LDS when the alphabet has <= 64 symbols, global atomics otherwise);
+//   1. histogram with LDS atomics (order-1: pair counts remapped to an
+//      (alphabet + 1) x alphabet table in LDS when that fits TAB_CAP entries,
+//      i.e. up to 42 symbols, global atomics otherwise);
 //   2. normalize (model.rs:213-264) as wave reductions -- the first index with
 //      the strictly greatest frequency absorbs the residual, exactly as the
 //      reference's sequential scan does;
@@ -28,24 +29,36 @@
 
 namespace {
 
-constexpr u32 DENSE_COLS = 64;
-constexpr u32 DENSE_ROWS = 65;           // 64 dense symbols + context 0 when absent
-constexpr u32 TAB_ENTRIES = DENSE_ROWS * DENSE_COLS;
+// Dense order-1 table: rows = the alphabet's symbols plus one row for
+// context 0 when symbol 0 is absent, columns = the alphabet.  Its capacity
+// keeps the whole shared block within 20 KiB, so two waves fit per SIMD.
+constexpr u32 TAB_CAP = 1808;
 constexpr u32 CH = 2048;                 // walk staging chunk (positions)
 constexpr u32 CH_DW = CH / 4 + 1;        // + 4 leading bytes (context of the first)
 #ifndef WALK_BATCH
 #define WALK_BATCH 8                      // rounds whose table entries are read together
 #endif
+constexpr u32 HB = 8;                    // histogram: dwords per lane loaded together
 
 struct EncShared {
-    u32 fc[TAB_ENTRIES];    // counts, then f << 12 | cum
-    u32 mg[TAB_ENTRIES];    // reciprocal magic per entry
+    u32 fc[TAB_CAP];        // counts, then f << 12 | cum
+    u32 mg[TAB_CAP];        // reciprocal magic per entry
     u32 hist[256];
     u8 colmap[256];         // symbol -> dense column (order-1 dense mode)
     u8 rowmap[256];         // context -> dense row
     u8 ctx_of[256];         // k-th present context (ascending)
     u32 stage[2][CH_DW + 12];   // + slack: lanes past the chunk read inside it
 };
+static_assert(sizeof(EncShared) <= 20480, "two one-wave workgroups per SIMD need <= 20 KiB of LDS each");
+
+// Workspace of one wave (WS_PER_WAVE bytes): order-1 counts, f << 12 | cum
+// and magics for alphabets past the dense table (256 KiB each), then the
+// 128-byte line the walk's non-emitting lanes store into.
+constexpr u64 WS_TRASH = 3ull * 256 * 256 * 4;
+static_assert(WS_TRASH + 128 <= WS_PER_WAVE, "encode workspace");
+
+// Alphabet sizes whose dense table fits: (a + 1) * a entries.
+__device__ __forceinline__ bool dense_fits(u32 a) { return (a + 1) * a <= TAB_CAP; }
 
 // Bytes of p[0..3] that fall in [begin, end), others zero.
 __device__ __forceinline__ u32 load4b(const u8 *p, const u8 *begin, const u8 *end)
@@ -152,6 +165,20 @@ __device__ __forceinline__ void stage_fetch(const u8 *in, u64 n, u64 p0, u32 (&r
     }
 }
 
+// Dense order-1: every staged byte becomes its dense id (rowmap).  A present
+// symbol has the same row and column id, and the zero fill before position 0
+// becomes context 0's row, so a staged pair (context, symbol) indexes the
+// (alphabet + 1) x alphabet table directly.
+__device__ __forceinline__ void stage_remap(const EncShared &sh, u32 (&r)[9])
+{
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const u32 w = r[i];
+        r[i] = (u32)sh.rowmap[w & 0xff] | (u32)sh.rowmap[(w >> 8) & 0xff] << 8 |
+               (u32)sh.rowmap[(w >> 16) & 0xff] << 16 | (u32)sh.rowmap[w >> 24] << 24;
+    }
+}
+
 __device__ __forceinline__ void stage_put(u32 *buf, const u32 (&r)[9])
 {
 #pragma unroll
@@ -168,6 +195,7 @@ struct TabRef {
     const u8 *rowmap;
     const u8 *colmap;
     u32 mode;   // 0: order-0 (index = sym); 1: dense order-1; 2: global order-1
+    u32 cols;   // dense order-1: the alphabet size (row stride)
 };
 
 
@@ -175,7 +203,7 @@ template <u32 MODE>
 __device__ __forceinline__ u32 tab_index_t(const TabRef &t, u32 ctx, u32 sym)
 {
     if (MODE == 0) return sym;
-    if (MODE == 1) return (u32)t.rowmap[ctx] * DENSE_COLS + t.colmap[sym];
+    if (MODE == 1) return (u32)t.rowmap[ctx] * t.cols + t.colmap[sym];
     return ctx * 256 + sym;
 }
 
@@ -194,13 +222,16 @@ __device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &t
     // Lanes 32..63 address as lanes 0..31: every read stays within the
     // chunk's stage (at most 31 positions past its end, inside the slack).
     const u32 t = (u32)(r * NSTATES + (l & (NSTATES - 1)) - p0);
+    // Position 0's context byte is the stage's zero fill before the input:
+    // context 0 (scalar.rs:64, 85).  Mode 1 stages dense ids (stage_remap),
+    // so the entry index is row * cols + column directly.
     const u32 sym = sb[4 + t];
-    const u32 ctx = i > 0 ? sb[3 + t] : 0u;
+    const u32 ctx = sb[3 + t];
     u32 e;
     if (MODE == 0)
         e = sym;
     else if (MODE == 1)
-        e = (u32)sh.rowmap[ctx] * DENSE_COLS + sh.colmap[sym];
+        e = ctx * tr.cols + sym;
     else
         e = ctx * 256 + sym;
     e = act ? e : 0u;
@@ -211,29 +242,32 @@ __device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &t
 }
 
 // One encode step of every lane (scalar.rs:120-127 per lane; lanes emit in
-// descending position order within the round).  The round's k words go to
-// [wend - 2(cnt + k), wend - 2 cnt): a wave-uniform base plus 2 * rank.
+// descending position order within the round).  `wp` (wave-uniform) is the
+// lowest word emitted so far; the round's k words go to [wp - k, wp), the
+// emitting lane of rank r (lanes below it that emit) at wp - k + r, so the
+// highest lane's word comes first in the reference's reversed order.  Every
+// lane stores: a lane with no word writes its own slot of `trash` (64 u16 in
+// the wave's workspace), so the step has no branch and the state chain never
+// waits on an exec-mask change.
 typedef __attribute__((address_space(1))) u8 gu8;
 typedef __attribute__((address_space(1))) u16 gu16;
-__device__ __forceinline__ void round_step(u32 &x, u64 &cnt, u32 fcm, u32 mg, gu8 *wend, bool even)
+__device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg, gu16 *trash)
 {
     const u32 f = fcm >> 12, cum = fcm & 0xfff;
     // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73); inactive
-    // lanes carry f = 4096, which no 32-bit state reaches.
+    // lanes (and lanes 32..63) carry f = 4096, which no 32-bit state reaches.
     const bool need = (x >> 20) >= f;
+    // The same test as a 0/1 value (sign of f - 1 - (x >> 20), which lies in
+    // [-4096, 4095]), so the shift and the store address below are
+    // arithmetic, not selects the compiler would lower to exec-mask changes:
+    // the state chain stays on the VALU.
+    const u32 nb = (f - 1u - (x >> 20)) >> 31;
+    const u32 v = x >> (nb << 4);
     const u64 m = ballot(need);
-    const u32 k = (u32)__popcll(m);
-    if (need) {
-        // Emission index cnt + (k - 1 - rank): higher lanes first.
-        gu8 *w = wend - 2 * (cnt + k) + 2 * rank_below(m);
-        if (even) {
-            *(gu16 *)w = (u16)x;
-        } else {
-            w[0] = (u8)x;
-            w[1] = (u8)(x >> 8);
-        }
-    }
-    const u32 v = need ? (x >> 16) : x;
+    wp -= (u32)__popcll(m);
+    const u64 aw = (u64)(uintptr_t)(wp + __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+    const u64 at = (u64)(uintptr_t)(trash + lane_id());
+    *(gu16 *)(uintptr_t)(at ^ ((aw ^ at) & (0ull - (u64)nb))) = (u16)x;
     u32 q = __umulhi(v, mg);
     u32 rr = v - q * f;
     if (rr >= f) {
@@ -241,22 +275,23 @@ __device__ __forceinline__ void round_step(u32 &x, u64 &cnt, u32 fcm, u32 mg, gu
         rr -= f;
     }
     x = (q << SCALE_BITS) + rr + cum;
-    cnt += k;
 }
 
-// The reverse walk over all n positions; returns the number of words.  The
-// table entries of WALK_BATCH rounds are read before their state updates,
-// so the LDS latency is paid once per batch on the state chain.
+// The reverse walk over all n positions; returns the number of words, which
+// end at `wend` (2-byte aligned).  The table entries of WALK_BATCH rounds are
+// read before their state updates, so the LDS latency is paid once per batch
+// on the state chain.
 template <u32 MODE>
-__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend_generic, u32 &x_out)
+__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u8 *trash_generic, u32 &x_out)
 {
-    gu8 *wend = (gu8 *)wend_generic;
-    const bool even = ((uintptr_t)wend_generic & 1) == 0;
+    gu16 *const trash = (gu16 *)trash_generic;
+    gu16 *const wtop = (gu16 *)wend;
+    gu16 *wp = wtop;
     u32 x = RANS_L; // a register for the whole walk, written back at the end
-    u64 cnt = 0;
     const u64 nch = (n + CH - 1) / CH;
     u32 pf[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
+    if (MODE == 1) stage_remap(sh, pf);
     stage_put(sh.stage[(nch - 1) & 1], pf);
     __syncthreads();
     for (u64 ch = nch; ch-- > 0;) {
@@ -271,23 +306,24 @@ __device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *we
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) round_entry<MODE>(sh, tr, sb, p0, r - 1 - k, n, fcm[k], mg[k]);
 #pragma unroll
-            for (u32 k = 0; k < WALK_BATCH; k++) round_step(x, cnt, fcm[k], mg[k], wend, even);
+            for (u32 k = 0; k < WALK_BATCH; k++) round_step(x, wp, fcm[k], mg[k], trash);
             r -= WALK_BATCH;
         }
         while (r > r_lo) {
             u32 fcm, mg;
             round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);
-            round_step(x, cnt, fcm, mg, wend, even);
+            round_step(x, wp, fcm, mg, trash);
             r--;
         }
         if (ch > 0) {
+            if (MODE == 1) stage_remap(sh, pf);
             __syncthreads();
             stage_put(sh.stage[(ch - 1) & 1], pf);
             __syncthreads();
         }
     }
     x_out = x;
-    return cnt;
+    return (u64)(wtop - wp);
 }
 
 // Tail of every encode: states, then the words slid down behind them.
@@ -300,15 +336,38 @@ __device__ void finish(u8 *dst, u64 hdr, u8 *wend, u64 W, u32 x)
     wave_copy_fwd(dst + hdr, wend - 2 * W, 2 * W);
 }
 
-// Histogram of in[0..n) into sh.hist (and, for order-1 dense mode, the pair
-// counts (context row, symbol column) into sh.fc).
+// Input dword t (bytes [4t, 4t+4)) as two aligned loads and a byte shift.
+// Both loads stay within [in, in + n) for t < bulk_dwords(n); the dword
+// A[t - 1] read for the byte before dword t (t >= 1) starts at or after the
+// aligned dword holding in[0].
+__device__ __forceinline__ u64 bulk_dwords(u64 n) { return n >= 8 ? (n - 8) / 4 + 1 : 0; }
+
+// Histogram of in[0..n) into sh.hist.
 __device__ void hist0(EncShared &sh, const u8 *in, u64 n)
 {
     const u32 l = lane_id();
     for (u32 s = l; s < 256; s += 64) sh.hist[s] = 0;
     __syncthreads();
-    const u64 nd = (n + 3) / 4;
-    for (u64 t = l; t < nd; t += 64) {
+    const u64 nd = (n + 3) / 4, nbulk = bulk_dwords(n);
+    const u32 ph = (u32)((uintptr_t)in & 3);
+    const u32 *A = (const u32 *)(in - ph);
+    // Bulk: HB dwords per lane are loaded before any of them is counted, so
+    // the loads of a batch are in flight together.
+    u64 t0 = 0;
+    for (; t0 + 64 * HB <= nbulk; t0 += 64 * HB) {
+        u32 w[HB];
+#pragma unroll
+        for (u32 u = 0; u < HB; u++) {
+            const u64 t = t0 + 64 * u + l;
+            w[u] = __builtin_amdgcn_alignbyte(A[t + 1], A[t], ph);
+        }
+#pragma unroll
+        for (u32 u = 0; u < HB; u++)
+#pragma unroll
+            for (u32 b = 0; b < 4; b++) atomicAdd(&sh.hist[(w[u] >> (8 * b)) & 0xff], 1u);
+    }
+    // The rest (fewer than 64 * HB + 2 dwords), one dword per lane at a time.
+    for (u64 t = t0 + l; t < nd; t += 64) {
         const u32 w = load4(in + 4 * t, in + n);
         const u32 nb = (4 * t + 4 <= n) ? 4 : (u32)(n - 4 * t);
         for (u32 b = 0; b < nb; b++) atomicAdd(&sh.hist[(w >> (8 * b)) & 0xff], 1u);
@@ -316,22 +375,53 @@ __device__ void hist0(EncShared &sh, const u8 *in, u64 n)
     __syncthreads();
 }
 
-// Pair counts: row(ctx) x col(sym) for every position (ctx of 0 is 0).
+// Pair counts: row(ctx) x col(sym) for every position (ctx of 0 is 0),
+// with the loads batched as in hist0.
 template <bool DENSE>
-__device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts)
+__device__ __forceinline__ void count_pair(EncShared &sh, u32 *gcounts, u32 cols, u32 prev, u32 s)
+{
+    if (DENSE)
+        atomicAdd(&sh.fc[(u32)sh.rowmap[prev] * cols + sh.colmap[s]], 1u);
+    else
+        atomicAdd(&gcounts[prev * 256 + s], 1u);
+}
+
+template <bool DENSE>
+__device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols)
 {
     const u32 l = lane_id();
-    const u64 nd = (n + 3) / 4;
-    for (u64 t = l; t < nd; t += 64) {
+    const u64 nd = (n + 3) / 4, nbulk = bulk_dwords(n);
+    const u32 ph = (u32)((uintptr_t)in & 3);
+    const u32 *A = (const u32 *)(in - ph);
+    u64 t0 = 0;
+    for (; t0 + 64 * HB <= nbulk; t0 += 64 * HB) {
+        u32 w[HB], pv[HB];
+#pragma unroll
+        for (u32 u = 0; u < HB; u++) {
+            const u64 t = t0 + 64 * u + l;
+            const u32 a0 = A[t];
+            w[u] = __builtin_amdgcn_alignbyte(A[t + 1], a0, ph);
+            // Byte 4t - 1: the top byte of input bytes [4t - 4, 4t).
+            pv[u] = t > 0 ? __builtin_amdgcn_alignbyte(a0, A[t - 1], ph) >> 24 : 0u;
+        }
+#pragma unroll
+        for (u32 u = 0; u < HB; u++) {
+            u32 prev = pv[u];
+#pragma unroll
+            for (u32 b = 0; b < 4; b++) {
+                const u32 s = (w[u] >> (8 * b)) & 0xff;
+                count_pair<DENSE>(sh, gcounts, cols, prev, s);
+                prev = s;
+            }
+        }
+    }
+    for (u64 t = t0 + l; t < nd; t += 64) {
         const u32 w = load4(in + 4 * t, in + n);
         u32 prev = t > 0 ? in[4 * t - 1] : 0u;
         const u32 nb = (4 * t + 4 <= n) ? 4 : (u32)(n - 4 * t);
         for (u32 b = 0; b < nb; b++) {
             const u32 s = (w >> (8 * b)) & 0xff;
-            if (DENSE)
-                atomicAdd(&sh.fc[(u32)sh.rowmap[prev] * DENSE_COLS + sh.colmap[s]], 1u);
-            else
-                atomicAdd(&gcounts[prev * 256 + s], 1u);
+            count_pair<DENSE>(sh, gcounts, cols, prev, s);
             prev = s;
         }
     }
@@ -357,7 +447,9 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         out_len = 9;
         return 0;
     }
-    u8 *wend = o + jb.dst_cap;
+    // Words are written downwards from the end of the region, 2-byte
+    // aligned (the bound leaves 16 spare bytes), then slid behind the header.
+    u8 *wend = (u8 *)((uintptr_t)(o + jb.dst_cap) & ~(uintptr_t)1);
     hist0(sh, in, n);
 
     if (order == 0) {
@@ -373,9 +465,9 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             cum += f[i];
         }
         __syncthreads();
-        TabRef tr{sh.fc, sh.mg, sh.rowmap, sh.colmap, 0};
+        TabRef tr{sh.fc, sh.mg, sh.rowmap, sh.colmap, 0, 0};
         u32 x;
-        const u64 W = walk<0>(sh, in, n, tr, wend, x);
+        const u64 W = walk<0>(sh, in, n, tr, wend, ws + WS_TRASH, x);
         finish(o, O0_HDR, wend, W, x);
         out_len = O0_HDR + 2 * W;
         return 0;
@@ -385,7 +477,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     u32 pres = 0;
     for (int i = 0; i < 4; i++) pres += sh.hist[4 * l + i] != 0;
     const u32 asz = uni(wave_sum(pres));
-    const bool dense = asz <= DENSE_COLS;
+    const bool dense = dense_fits(asz);
     u32 *gcounts = (u32 *)ws;
     u32 *gfc = (u32 *)(ws + 256ull * 256 * 4);
     u32 *gmg = (u32 *)(ws + 2 * 256ull * 256 * 4);
@@ -396,16 +488,16 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             const u32 s = 4 * l + i;
             const bool p = sh.hist[s] != 0;
             sh.colmap[s] = p ? (u8)d : (u8)0;
-            sh.rowmap[s] = p ? (u8)d : (u8)(DENSE_ROWS - 1);
+            sh.rowmap[s] = p ? (u8)d : (u8)asz;
             d += p;
         }
-        for (u32 e = l; e < TAB_ENTRIES; e += 64) sh.fc[e] = 0;
+        for (u32 e = l; e < (asz + 1) * asz; e += 64) sh.fc[e] = 0;
         __syncthreads();
-        hist1<true>(sh, in, n, nullptr);
+        hist1<true>(sh, in, n, nullptr, asz);
     } else {
         for (u64 e = l; e < 256 * 256; e += 64) gcounts[e] = 0;
         __threadfence();
-        hist1<false>(sh, in, n, gcounts);
+        hist1<false>(sh, in, n, gcounts, 0);
         __threadfence();
     }
     // Present contexts (row sum > 0), ascending.
@@ -417,7 +509,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             const u32 row = sh.rowmap[ctx];
             const bool valid = sh.hist[ctx] != 0 || ctx == 0;
             if (valid)
-                for (u32 cI = 0; cI < DENSE_COLS; cI++) any |= sh.fc[row * DENSE_COLS + cI];
+                for (u32 cI = 0; cI < asz; cI++) any |= sh.fc[row * asz + cI];
         } else {
             for (u32 sI = 0; sI < 256; sI++) any |= gcounts[ctx * 256 + sI];
         }
@@ -444,7 +536,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         for (int i = 0; i < 4; i++) {
             const u32 s = 4 * l + i;
             if (dense)
-                c[i] = sh.hist[s] ? sh.fc[(u32)sh.rowmap[ctx] * DENSE_COLS + sh.colmap[s]] : 0u;
+                c[i] = sh.hist[s] ? sh.fc[(u32)sh.rowmap[ctx] * asz + sh.colmap[s]] : 0u;
             else
                 c[i] = gcounts[ctx * 256 + s];
         }
@@ -457,7 +549,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             const u32 s = 4 * l + i;
             if (dense) {
                 if (sh.hist[s]) {
-                    const u32 e = (u32)sh.rowmap[ctx] * DENSE_COLS + sh.colmap[s];
+                    const u32 e = (u32)sh.rowmap[ctx] * asz + sh.colmap[s];
                     sh.fc[e] = (f[i] << 12) | cum;
                     sh.mg[e] = magic_of(f[i]);
                 }
@@ -470,10 +562,11 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         __syncthreads();
     }
     if (!dense) __threadfence();
-    TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u}
-                      : TabRef{gfc, gmg, sh.rowmap, sh.colmap, 2u};
+    TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u, asz}
+                      : TabRef{gfc, gmg, sh.rowmap, sh.colmap, 2u, 0};
     u32 x;
-    const u64 W = dense ? walk<1>(sh, in, n, tr, wend, x) : walk<2>(sh, in, n, tr, wend, x);
+    const u64 W = dense ? walk<1>(sh, in, n, tr, wend, ws + WS_TRASH, x)
+                        : walk<2>(sh, in, n, tr, wend, ws + WS_TRASH, x);
     finish(o, hdr, wend, W, x);
     out_len = hdr + 2 * W;
     return 0;
@@ -538,7 +631,12 @@ __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, 
         const u32 j = order ? uni(order[w]) : w;
         const fqxv_rans_job jb = uni_job(jobs[j]);
         u64 ol = 0;
+        // Long blocks are the batch's critical path (the queue runs them
+        // first): their wave issues first on a shared SIMD.
+        const bool crit = jb.src_len >= (1u << 20);
+        if (crit) __builtin_amdgcn_s_setprio(3);
         const int err = best ? encode_best_job(sh, src, dst, jb, ol, myws) : encode_job(sh, src, dst, jb, ol, myws);
+        if (crit) __builtin_amdgcn_s_setprio(0);
         if (lane_id() == 0) {
             status[j] = err;
             out_len[j] = err ? 0 : ol;

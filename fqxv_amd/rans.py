@@ -25,7 +25,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tune
 
 # Declared lengths above this are treated like a failed `try_reserve_exact`
 # (model.rs:199-205): the decode still validates tables and states first.
@@ -225,7 +225,8 @@ class Coder:
             dst_off[1:] = np.cumsum(caps)[:-1]
         dst = torch.empty(int(caps.sum()) + 64, dtype=torch.uint8, device=src.device)
         jobs = self.jobs_array(src_off, src_len, dst_off, caps, order=orders)
-        order_np = self.lpt_order(src_len) if lpt else None
+        # Largest estimated encode time first (order-1 bytes cost more).
+        order_np = tune.queue_order(self, "lpt-cost", src_len, orders, "encode") if lpt else None
         status, out_len, keep = self._run_batch(self._L.fqxv_rans_encode_batch, src, dst, jobs, order_np, stream)
         self._keep = keep
         return BatchResult(dst, dst_off, out_len, status)

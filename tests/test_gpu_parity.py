@@ -253,9 +253,9 @@ def test_encode_best_batch_matches_oracle(coder, oracle):
 
 @pytest.mark.parametrize("k", [64, 65, 100, 128, 129, 192, 193, 255, 256])
 def test_order1_wide_alphabets(coder, oracle, k):
-    """Order-1 with more than 64 contexts (each candidate vector spread over
-    up to four VGPRs): device encode == oracle, device decode == input, for
-    K on both sides of every 64-context boundary."""
+    """Order-1 with more than 64 contexts (four contexts packed per lane of
+    each gathered column): device encode == oracle, device decode == input,
+    for K on both sides of every 64-context boundary."""
     import torch
     rng = np.random.default_rng(900 + k)
     n = 40000 + k
@@ -277,6 +277,70 @@ def test_order1_wide_alphabets(coder, oracle, k):
     out = r.dst.cpu().numpy()
     assert out[:len(data)].tobytes() == data
     assert out[int(r.dst_off[1]):int(r.dst_off[1]) + len(short)].tobytes() == short
+
+
+@pytest.mark.parametrize("k", [1, 2, 40, 41, 42, 43, 44, 63, 64])
+@pytest.mark.parametrize("base", [0, 33])
+def test_order1_encode_dense_capacity(coder, oracle, k, base):
+    """Order-1 encode on both sides of the dense LDS table's capacity (an
+    (alphabet + 1) x alphabet table of at most 1808 entries: 42 symbols) and
+    with symbol 0 present or absent (absent: context 0 takes the extra row).
+    Single and batched device encode == oracle; batched decode == input."""
+    import torch
+    rng = np.random.default_rng(1300 + 2 * k + base)
+    blocks = []
+    for n in (5 + k, 3000 + k, 70000 + k):
+        b = (base + rng.integers(0, k, n)).astype(np.uint8)
+        b[: min(k, n)] = base + np.arange(min(k, n))  # every symbol occurs
+        blocks.append(b.tobytes())
+    for b in blocks:
+        assert coder.encode(b, 1) == oracle.encode(b, 1)
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    offs = np.zeros(len(blocks), dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(blocks) + bytes(64), dtype=np.uint8).copy()).cuda()
+    r = coder.encode_batch(arena, offs, sizes, [1] * len(blocks))
+    torch.cuda.synchronize()
+    assert np.all(r.status.cpu().numpy() == 0)
+    ol = r.out_len.cpu().numpy()
+    host = r.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        assert host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes() == oracle.encode(b, 1), j
+    d = coder.decode_batch(r.dst, r.dst_off, ol, sizes)
+    torch.cuda.synchronize()
+    assert np.all(d.status.cpu().numpy() == 0)
+    assert torch.equal(d.dst[: int(sizes.sum())], arena[: int(sizes.sum())])
+
+
+@pytest.mark.parametrize("shift", [0, 1, 2, 3])
+def test_encode_histogram_edges(coder, oracle, shift):
+    """Block sizes around the histogram's 2 KiB load batches and its 8-byte
+    bulk limit, at every source alignment, both orders: batched device
+    encode == oracle."""
+    import torch
+    rng = np.random.default_rng(77 + shift)
+    sizes = [0, 1, 4, 7, 8, 9, 12, 2047, 2048, 2049, 2055, 2056, 2057, 2060, 4103, 4104, 4105, 6000, 20483]
+    blocks = []
+    for i, n in enumerate(sizes):
+        alpha = 3 + (i % 5) * 9  # 3 .. 39 symbols
+        blocks.append((33 + rng.integers(0, alpha, n)).astype(np.uint8).tobytes())
+    for order in (0, 1):
+        lens = np.array([len(b) for b in blocks], dtype=np.uint64)
+        offs = np.zeros(len(blocks), dtype=np.uint64)
+        offs[1:] = np.cumsum(lens + 3)[:-1]
+        offs += shift
+        buf = bytearray(int(offs[-1] + lens[-1]) + 64)
+        for o, b in zip(offs, blocks):
+            buf[int(o):int(o) + len(b)] = b
+        arena = torch.from_numpy(np.frombuffer(bytes(buf), dtype=np.uint8).copy()).cuda()
+        r = coder.encode_batch(arena, offs, lens, [order] * len(blocks))
+        torch.cuda.synchronize()
+        assert np.all(r.status.cpu().numpy() == 0)
+        ol = r.out_len.cpu().numpy()
+        host = r.dst.cpu().numpy()
+        for j, b in enumerate(blocks):
+            got = host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes()
+            assert got == oracle.encode(b, order), (order, len(b))
 
 
 def test_order1_wide_malformed_same_error(coder, oracle):

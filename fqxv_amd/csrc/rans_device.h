@@ -147,10 +147,13 @@ __device__ __forceinline__ void store_bytes(u8 *p, u32 v, u32 nbytes)
 }
 
 // Wave-cooperative forward copy of `len` bytes with arbitrary alignment of
-// both ends.  Safe for overlapping ranges when dst + 16 <= src (each
-// iteration reads its whole source window before any lane stores).
+// both ends.  Safe for overlapping ranges when dst <= src: each batch reads
+// its whole source window (8 dwords per lane, plus the unaligned neighbour)
+// before any lane stores, and a batch's stores end below the next batch's
+// source (dst + 2048 (b + 1) <= src + 2048 (b + 1)).
 __device__ inline void wave_copy_fwd(u8 *dst, const u8 *src, u64 len)
 {
+    constexpr u32 CB = 8;
     const u32 l = lane_id();
     const u8 *send = src + len;
     // Destination head up to the next 4-byte boundary.
@@ -162,7 +165,25 @@ __device__ inline void wave_copy_fwd(u8 *dst, const u8 *src, u64 len)
     len -= head;
     u32 *d32 = (u32 *)dst;
     const u64 nd = len / 4;
-    for (u64 base = 0; base < nd; base += 64) {
+    // Bulk: source dword k as two aligned loads and a byte shift, both
+    // inside [src, send) while 4k + 8 <= len.
+    const u64 nbulk = len >= 8 ? (len - 8) / 4 + 1 : 0;
+    const u32 ph = (u32)((uintptr_t)src & 3);
+    const u32 *A = (const u32 *)(src - ph);
+    u64 base = 0;
+    for (; base + 64 * CB <= nbulk; base += 64 * CB) {
+        u32 v[CB];
+#pragma unroll
+        for (u32 u = 0; u < CB; u++) {
+            const u64 k = base + 64 * u + l;
+            v[u] = __builtin_amdgcn_alignbyte(A[k + 1], A[k], ph);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (u32 u = 0; u < CB; u++) d32[base + 64 * u + l] = v[u];
+    }
+    for (; base < nd; base += 64) {
         const u64 k = base + l;
         u32 v = 0;
         if (k < nd) v = load4(src + 4 * k, send);

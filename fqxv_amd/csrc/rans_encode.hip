@@ -406,12 +406,24 @@ __device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols
         }
 #pragma unroll
         for (u32 u = 0; u < HB; u++) {
-            u32 prev = pv[u];
+            if (DENSE) {
+                // A present symbol has the same row and column id, so each
+                // byte is mapped once; the byte before the dword through
+                // rowmap (context 0 of position 0 may be absent).
+                u32 id[5];
+                id[0] = sh.rowmap[pv[u]];
 #pragma unroll
-            for (u32 b = 0; b < 4; b++) {
-                const u32 s = (w[u] >> (8 * b)) & 0xff;
-                count_pair<DENSE>(sh, gcounts, cols, prev, s);
-                prev = s;
+                for (u32 b = 0; b < 4; b++) id[b + 1] = sh.colmap[(w[u] >> (8 * b)) & 0xff];
+#pragma unroll
+                for (u32 b = 0; b < 4; b++) atomicAdd(&sh.fc[id[b] * cols + id[b + 1]], 1u);
+            } else {
+                u32 prev = pv[u];
+#pragma unroll
+                for (u32 b = 0; b < 4; b++) {
+                    const u32 s = (w[u] >> (8 * b)) & 0xff;
+                    count_pair<DENSE>(sh, gcounts, cols, prev, s);
+                    prev = s;
+                }
             }
         }
     }

@@ -36,7 +36,7 @@ constexpr u32 TAB_CAP = 1808;
 constexpr u32 CH = 2048;                 // walk staging chunk (positions)
 constexpr u32 CH_DW = CH / 4 + 1;        // + 4 leading bytes (context of the first)
 #ifndef WALK_BATCH
-#define WALK_BATCH 8                      // rounds whose table entries are read together
+#define WALK_BATCH 16                     // rounds whose table entries are read together
 #endif
 constexpr u32 HB = 8;                    // histogram: dwords per lane loaded together
 

@@ -643,12 +643,7 @@ __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, 
         const u32 j = order ? uni(order[w]) : w;
         const fqxv_rans_job jb = uni_job(jobs[j]);
         u64 ol = 0;
-        // Long blocks are the batch's critical path (the queue runs them
-        // first): their wave issues first on a shared SIMD.
-        const bool crit = jb.src_len >= (1u << 20);
-        if (crit) __builtin_amdgcn_s_setprio(3);
         const int err = best ? encode_best_job(sh, src, dst, jb, ol, myws) : encode_job(sh, src, dst, jb, ol, myws);
-        if (crit) __builtin_amdgcn_s_setprio(0);
         if (lane_id() == 0) {
             status[j] = err;
             out_len[j] = err ? 0 : ol;

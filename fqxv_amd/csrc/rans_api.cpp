@@ -37,6 +37,8 @@ void fqxv_crc_const_fill(void *);
 hipError_t fqxv_launch_crc(const u8 *, const u64 *, const u64 *, const u32 *, const u32 *, u32, const u32 *, u32,
                            const void *, u32 *, u32 *, u32, hipStream_t);
 hipError_t fqxv_launch_perm_join(const u8 *, u64, u32 *, hipStream_t);
+hipError_t fqxv_launch_pack_jobs(const u8 *, const fqxv_rans_job *, const u64 *, const i32 *, const u64 *, u32, u8 *,
+                                 hipStream_t);
 hipError_t fqxv_launch_xxh3(const u8 *, const u64 *, const u64 *, const u64 *, u32, u64, u64 *, u64 *,
                             hipStream_t);
 
@@ -414,6 +416,75 @@ int fqxv_run_jobs_host(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_
         (dst_total && hipMemcpyAsync(h_dst, c->d_out, dst_total, hipMemcpyDeviceToHost, st) != hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
         return FQXV_RANS_ERR_DEVICE;
+    return FQXV_RANS_OK;
+}
+
+int fqxv_encode_jobs_host_packed(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_len,
+                                 const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total,
+                                 uint8_t **h_packed, uint64_t *h_pack_off, int32_t *h_status, uint64_t *h_out_len)
+{
+    if (!c || !h_packed || (src_len && !h_src) || (count && (!h_jobs || !h_pack_off || !h_status || !h_out_len)))
+        return FQXV_RANS_ERR_INVALID_ARG;
+    *h_packed = nullptr;
+    std::vector<u64> cost(count);
+    for (u32 i = 0; i < count; i++) {
+        const fqxv_rans_job &j = h_jobs[i];
+        if (j.src_off + j.src_len > src_len || j.dst_off + j.dst_cap > dst_total) return FQXV_RANS_ERR_INVALID_ARG;
+        cost[i] = j.dst_cap + j.src_len;
+    }
+    std::vector<u32> order(count);
+    if (count) fqxv_rans_lpt_order(cost.data(), count, order.data());
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    const u32 grid = std::min<u32>(std::max<u32>(count, 1), c->enc_waves);
+    const u64 jb = sizeof(fqxv_rans_job) * (u64)count, ob = 4ull * count, sb = 4ull * count, lb = 8ull * count;
+    int rc;
+    if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, src_len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dst_total)) ||
+        (rc = ensure_buf(&c->job_buf, &c->job_cap, 2 * lb + jb + ob + sb)) || (rc = ensure_ws(c, grid)))
+        return rc;
+    u64 *d_len = (u64 *)c->job_buf;
+    u64 *d_poff = d_len + count;
+    fqxv_rans_job *d_jobs = (fqxv_rans_job *)(d_poff + count);
+    u32 *d_order = (u32 *)(d_jobs + count);
+    i32 *d_st = (i32 *)(d_order + count);
+    hipStream_t st = c->stream;
+    if (count) {
+        if ((src_len && hipMemcpyAsync(c->d_in, h_src, src_len, hipMemcpyHostToDevice, st) != hipSuccess) ||
+            hipMemcpyAsync(d_jobs, h_jobs, jb, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_order, order.data(), ob, hipMemcpyHostToDevice, st) != hipSuccess ||
+            fqxv_launch_encode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len, c->ws,
+                                     grid, st, 1) != hipSuccess ||
+            hipMemcpyAsync(h_status, d_st, sb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(h_out_len, d_len, lb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return FQXV_RANS_ERR_DEVICE;
+    }
+    // Packed offsets of the successful jobs, in job order.
+    u64 total = 0;
+    for (u32 i = 0; i < count; i++) {
+        h_pack_off[i] = total;
+        if (h_status[i] == 0) {
+            if (h_out_len[i] > h_jobs[i].dst_cap) return FQXV_RANS_ERR_DEVICE;
+            total += h_out_len[i];
+        }
+    }
+    u8 *h = (u8 *)malloc(std::max<u64>(total, 1));
+    if (!h) return FQXV_RANS_ERR_DEVICE;
+    // The sources are consumed: the packed outputs are staged in d_in.
+    if (total) {
+        if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, total))) {
+            free(h);
+            return rc;
+        }
+        if (hipMemcpyAsync(d_poff, h_pack_off, lb, hipMemcpyHostToDevice, st) != hipSuccess ||
+            fqxv_launch_pack_jobs(c->d_out, d_jobs, d_len, d_st, d_poff, count, c->d_in, st) != hipSuccess ||
+            hipMemcpyAsync(h, c->d_in, total, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            free(h);
+            return FQXV_RANS_ERR_DEVICE;
+        }
+    }
+    *h_packed = h;
     return FQXV_RANS_OK;
 }
 

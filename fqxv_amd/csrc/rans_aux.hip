@@ -1,4 +1,5 @@
-// Byte-plane transposes around the coder for the reorder permutation.
+// Byte-plane transposes around the coder for the reorder permutation, and
+// the packing of batched outputs for host callers.
 //
 // crates/fqxv/src/container/reorder.rs:597-607 stores the global permutation
 // (n u32 read indices) as four byte planes of n bytes each -- plane k holds
@@ -78,7 +79,29 @@ u32 aux_grid(u64 n)
     return (u32)(blocks < 65536 ? (blocks ? blocks : 1) : 65536);
 }
 
+// One wave per job (grid-stride): job i's output, out_len[i] bytes at
+// jobs[i].dst_off of the arena, to packed + pack_off[i]; a failed job copies
+// nothing.  The arena and the packed buffer do not overlap.
+__global__ __launch_bounds__(64) void k_pack_jobs(const u8 *__restrict__ arena, const fqxv_rans_job *__restrict__ jobs,
+                                                  const u64 *__restrict__ out_len, const i32 *__restrict__ status,
+                                                  const u64 *__restrict__ pack_off, u32 count, u8 *__restrict__ packed)
+{
+    for (u32 i = blockIdx.x; i < count; i += gridDim.x) {
+        if (uni((u32)status[i]) != 0) continue;
+        wave_copy_fwd(packed + uni64(pack_off[i]), arena + uni64(jobs[i].dst_off), uni64(out_len[i]));
+    }
+}
+
 } // namespace
+
+hipError_t fqxv_launch_pack_jobs(const u8 *arena, const fqxv_rans_job *jobs, const u64 *out_len, const i32 *status,
+                                 const u64 *pack_off, u32 count, u8 *packed, hipStream_t st)
+{
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_jobs, dim3(count < 4096 ? count : 4096), dim3(64), 0, st, arena, jobs, out_len, status,
+                       pack_off, count, packed);
+    return hipGetLastError();
+}
 
 hipError_t fqxv_launch_perm_split(const u32 *perm, u64 n, u8 *planes, hipStream_t st)
 {

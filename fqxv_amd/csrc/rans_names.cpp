@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -580,13 +581,17 @@ static int names_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
         for (u32 c = 0; c < NCOL; c++)
             for (const auto &pl : P.planes[c]) add(pl);
     }
-    std::vector<u8> raw(src_total), enc(dst_total);
+    // Sub-streams back to back (16-byte aligned, the padding zeroed); the
+    // encoded streams come back packed, not in their bound-sized regions.
+    std::unique_ptr<u8[]> raw(new u8[std::max<u64>(src_total, 1)]);
     {
         size_t j = 0;
         for (Parts &P : B->e) {
             if (P.status) continue;
             auto put = [&](const std::vector<u8> &st) {
-                if (!st.empty()) memcpy(raw.data() + jobs[j].src_off, st.data(), st.size());
+                u8 *d = raw.get() + jobs[j].src_off;
+                if (!st.empty()) memcpy(d, st.data(), st.size());
+                memset(d + st.size(), 0, ((st.size() + 15) & ~15ull) - st.size());
                 j++;
             };
             for (u32 k = 0; k < NFLAT; k++) put(P.flat[k]);
@@ -595,12 +600,12 @@ static int names_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
         }
     }
     std::vector<i32> st(jobs.size());
-    std::vector<u64> ol(jobs.size());
-    if (!jobs.empty()) {
-        const int rc = fqxv_run_jobs_host(ctx, 2, raw.data(), src_total, jobs.data(), (u32)jobs.size(), dst_total,
-                                          enc.data(), st.data(), ol.data());
-        if (rc) return rc;
-    }
+    std::vector<u64> ol(jobs.size()), poff(jobs.size());
+    u8 *packed = nullptr;
+    const int rc = fqxv_encode_jobs_host_packed(ctx, raw.get(), src_total, jobs.data(), (u32)jobs.size(), dst_total,
+                                                &packed, poff.data(), st.data(), ol.data());
+    if (rc) return rc;
+    std::unique_ptr<u8, decltype(&free)> enc(packed, &free);
     // Frame every block (lib.rs:515-537).
     std::vector<size_t> first(count);
     {
@@ -626,7 +631,7 @@ static int names_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
                 return;
             }
             put_varint(o, ol[j]);
-            o.insert(o.end(), enc.data() + jobs[j].dst_off, enc.data() + jobs[j].dst_off + ol[j]);
+            o.insert(o.end(), enc.get() + poff[j], enc.get() + poff[j] + ol[j]);
             j++;
         };
         for (u32 k = 0; k < NFLAT && !P.status; k++) emit();

@@ -502,13 +502,14 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
             dst_total += cap;
         }
     }
-    // Phase 2: one batched launch over all sub-streams.
-    std::vector<u8> dec(dst_total);
+    // Phase 2: one batched launch over all sub-streams.  The copy back
+    // overwrites the whole arena, so the host buffer is not zero-filled.
+    std::unique_ptr<u8[]> dec(new u8[std::max<u64>(dst_total, 1)]);
     std::vector<i32> job_st(jobs.size());
     std::vector<u64> job_len(jobs.size());
     if (!jobs.empty()) {
         const int rc = fqxv_run_jobs_host(ctx, 0, src + lo, hi - lo, jobs.data(), (u32)jobs.size(), dst_total,
-                                          dec.data(), job_st.data(), job_len.data());
+                                          dec.get(), job_st.data(), job_len.data());
         if (rc) return rc;
     }
     // Phase 3, threads across streams.
@@ -519,7 +520,7 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
         for (u32 i; (i = next.fetch_add(1)) < count;) {
             Stream &S = B->s[i];
             try {
-                S.status = replay(S, dec.data(), job_st.data(), job_len.data(), job_off.data());
+                S.status = replay(S, dec.get(), job_st.data(), job_len.data(), job_off.data());
             } catch (const std::bad_alloc &) {
                 S.status = FQXV_NAMES_ERR_ALLOC; // the reference's fallible reserves (lib.rs:150-152, 244-245)
             }

@@ -167,11 +167,25 @@ __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_blocks(const u8 *__restric
 #pragma unroll
             for (u32 s = 0; s < 16; s++) acc += stripe_lane(wv[s], (u64)__shfl_xor((long long)wv[s], 1, 8), key[s]);
         } else {
-#pragma unroll 4
-            for (u32 s = 0; s < 16; s++) {
-                const u64 v = ld64(p + 64 * s + 8 * i);
-                acc += stripe_lane(v, (u64)__shfl_xor((long long)v, 1, 8), key[s]);
+            // Word 16s + 2i of the block from aligned dwords: two when the
+            // block is 4-byte aligned, else three and two byte shifts.  Each
+            // dword read holds at least one byte of the block.
+            const u32 ph = (u32)((uintptr_t)p & 3);
+            const u32 *A = (const u32 *)(p - ph) + 2 * i;
+            u64 wv[16];
+            if (ph == 0) {
+#pragma unroll
+                for (u32 s = 0; s < 16; s++) wv[s] = (u64)A[16 * s] | ((u64)A[16 * s + 1] << 32);
+            } else {
+#pragma unroll
+                for (u32 s = 0; s < 16; s++) {
+                    const u32 a0 = A[16 * s], a1 = A[16 * s + 1], a2 = A[16 * s + 2];
+                    wv[s] = (u64)__builtin_amdgcn_alignbyte(a1, a0, ph) |
+                            ((u64)__builtin_amdgcn_alignbyte(a2, a1, ph) << 32);
+                }
             }
+#pragma unroll
+            for (u32 s = 0; s < 16; s++) acc += stripe_lane(wv[s], (u64)__shfl_xor((long long)wv[s], 1, 8), key[s]);
         }
         sums[it * 8 + i] = acc;
     }

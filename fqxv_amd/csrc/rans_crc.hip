@@ -22,6 +22,9 @@ namespace {
 constexpr u32 CRC_CHUNK_LOG = 16;                 // 64 KiB per wave
 constexpr u32 CRC_SEG = 1024;                     // bytes per lane
 constexpr u32 CRC_OPS = 48;                       // operators for 2^0 .. 2^47 bytes
+#ifndef CRC_GROUP
+#define CRC_GROUP 4                               // 16-byte loads per group
+#endif
 
 struct CrcConst {
     u32 table[8][256];                            // slicing-by-8
@@ -73,8 +76,27 @@ __global__ __launch_bounds__(64) void k_crc_chunks(const u8 *__restrict__ src, c
         const u8 *p = src + off[r] + cstart + a;
         u32 crc = 0;
         u32 i = 0;
-        // Head bytes up to 8-byte alignment, 8-byte steps, tail bytes.
-        while (i < n && (((uintptr_t)(p + i)) & 7)) crc = t[0][(crc ^ p[i++]) & 0xff] ^ (crc >> 8);
+        // Head bytes up to 16-byte alignment; groups of CRC_GROUP 16-byte
+        // loads issued together (each lane walks its own segment, so a load
+        // instruction touches 64 lines: taking a line's bytes in few loads
+        // keeps it from being fetched again after an L1 eviction); then
+        // 16-byte, 8-byte and single-byte tails.
+        while (i < n && (((uintptr_t)(p + i)) & 15)) crc = t[0][(crc ^ p[i++]) & 0xff] ^ (crc >> 8);
+        for (; i + 16 * CRC_GROUP <= n; i += 16 * CRC_GROUP) {
+            uint4 w[CRC_GROUP];
+#pragma unroll
+            for (u32 g = 0; g < CRC_GROUP; g++) w[g] = *(const uint4 *)(p + i + 16 * g);
+#pragma unroll
+            for (u32 g = 0; g < CRC_GROUP; g++) {
+                crc = slice8(t, crc, w[g].x, w[g].y);
+                crc = slice8(t, crc, w[g].z, w[g].w);
+            }
+        }
+        for (; i + 16 <= n; i += 16) {
+            const uint4 w = *(const uint4 *)(p + i);
+            crc = slice8(t, crc, w.x, w.y);
+            crc = slice8(t, crc, w.z, w.w);
+        }
         for (; i + 8 <= n; i += 8) {
             const uint2 w = *(const uint2 *)(p + i);
             crc = slice8(t, crc, w.x, w.y);

@@ -310,8 +310,7 @@ def bench_config(coder, dev, oracle, name, check):
     c_total = int(comp_len.sum())
     idx = list(range(nb))
     # plain form: encode then decode, one launch per block
-    caps = np.array([coder.encode_bound(int(n), int(o)) for n, o in zip(corpus.sizes, corpus.orders)],
-                    dtype=np.uint64)
+    caps = coder.encode_bounds(corpus.sizes, corpus.orders)
     e_off = np.zeros_like(caps)
     e_off[1:] = np.cumsum(caps)[:-1]
     edst = torch.empty(int(caps.sum()) + 64, dtype=torch.uint8, device=dev)

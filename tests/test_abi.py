@@ -75,3 +75,21 @@ def test_product_never_imports_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h", ".c")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower().replace("the oracle", ""), f
+
+
+def test_vectorised_encode_bounds_match_the_library(L):
+    """Coder.encode_bounds / encode_best_bounds (numpy, used for whole
+    batches) equal fqxv_rans_encode_bound / _encode_best_bound per block."""
+    import numpy as np
+    from fqxv_amd.rans import Coder
+    rng = np.random.default_rng(5)
+    ns = [0, 1, 2, 8, 9, 255, 256, 257, 1000, 1 << 20, (1 << 32) - 1, 1 << 33]
+    ns += [int(x) for x in rng.integers(0, 1 << 26, 200)]
+    for order in (0, 1):
+        got = Coder.encode_bounds(ns, [order] * len(ns))
+        assert [int(v) for v in got] == [int(L.fqxv_rans_encode_bound(n, order)) for n in ns]
+    mixed = [int(o) for o in rng.integers(0, 2, len(ns))]
+    got = Coder.encode_bounds(ns, mixed)
+    assert [int(v) for v in got] == [int(L.fqxv_rans_encode_bound(n, o)) for n, o in zip(ns, mixed)]
+    got = Coder.encode_best_bounds(ns)
+    assert [int(v) for v in got] == [int(L.fqxv_rans_encode_best_bound(n)) for n in ns]

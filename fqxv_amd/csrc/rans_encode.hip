@@ -440,8 +440,10 @@ __device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols
     __syncthreads();
 }
 
+// `have_hist`: sh.hist already holds this input's byte histogram (the
+// order-1 pass of encode_best_job follows the order-0 pass, which leaves it).
 __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb_in, u64 &out_len,
-                          u8 *ws)
+                          u8 *ws, bool have_hist = false)
 {
     const fqxv_rans_job jb = uni_job(jb_in);
     const u32 l = lane_id();
@@ -462,7 +464,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     // Words are written downwards from the end of the region, 2-byte
     // aligned (the bound leaves 16 spare bytes), then slid behind the header.
     u8 *wend = (u8 *)((uintptr_t)(o + jb.dst_cap) & ~(uintptr_t)1);
-    hist0(sh, in, n);
+    if (!have_hist) hist0(sh, in, n);
 
     if (order == 0) {
         u32 c[4], f[4];
@@ -615,7 +617,7 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
     int err = encode_job(sh, src, dst, j0, l0, ws);
     if (err) return err;
     __syncthreads();
-    err = encode_job(sh, src, dst, j1, l1, ws);
+    err = encode_job(sh, src, dst, j1, l1, ws, true);
     if (err) return err;
     const bool one = tie == 1 ? l1 <= l0 : l1 < l0;
     if (one) {

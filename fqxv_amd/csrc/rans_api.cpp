@@ -35,7 +35,9 @@ hipError_t fqxv_launch_perm_split(const u32 *, u64, u8 *, hipStream_t);
 size_t fqxv_crc_const_bytes();
 void fqxv_crc_const_fill(void *);
 hipError_t fqxv_launch_crc(const u8 *, const u64 *, const u64 *, const u32 *, const u32 *, u32, const u32 *, u32,
-                           const void *, u32 *, u32 *, u32, hipStream_t);
+                           const void *, u32 *, u32 *, u32, const u64 *, const u32 *, u32, const u32 *, u32 *,
+                           hipStream_t);
+uint32_t fqxv_crc_seg_chunks();
 hipError_t fqxv_launch_perm_join(const u8 *, u64, u32 *, hipStream_t);
 hipError_t fqxv_launch_pack_jobs(const u8 *, const fqxv_rans_job *, const u64 *, const i32 *, const u64 *, u32, u8 *,
                                  hipStream_t);
@@ -637,21 +639,49 @@ int fqxv_rans_crc32c_batch(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_
         }
     }
     const u64 ni = item_range.size();
-    // Device layout: off[count] len[count] (u64), first[count] item_range[ni]
-    // item_chunk[ni] raw[ni] (u32).
-    const u64 need = 16ull * count + 4ull * count + 12ull * ni + 64;
+    // A range of more than one segment (fqxv_crc_seg_chunks() chunks) is
+    // folded in two levels: its segments first (in parallel), then the
+    // segment values; otherwise one fold per range over its chunk values.
+    const u64 seg = fqxv_crc_seg_chunks();
+    std::vector<u64> seg_len;
+    std::vector<u32> seg_first, range_seg;
+    bool two_level = false;
+    for (u32 r = 0; r < count; r++) two_level |= ((len[r] + 65535) >> 16) > seg;
+    if (two_level) {
+        range_seg.resize(count);
+        for (u32 r = 0; r < count; r++) {
+            range_seg[r] = (u32)seg_len.size();
+            const u64 nch = (len[r] + 65535) >> 16;
+            for (u64 k = 0; k < nch; k += seg) {
+                const u64 b0 = k << 16, b1 = std::min<u64>((k + seg) << 16, len[r]);
+                seg_len.push_back(b1 - b0);
+                seg_first.push_back(first[r] + (u32)k);
+            }
+        }
+    }
+    const u64 ns = seg_len.size();
+    // Device layout: off[count] len[count] seg_len[ns] (u64), first[count]
+    // item_range[ni] item_chunk[ni] raw[ni] seg_first[ns] range_seg[count]
+    // seg_raw[ns] (u32).
+    const u64 need = 16ull * count + 8ull * ns + 4ull * count + 12ull * ni + 8ull * ns + 4ull * count + 64;
     int rc = ensure_buf(&c->crc_buf, &c->crc_cap, need);
     if (rc) return rc;
-    u64 *d_off = (u64 *)c->crc_buf, *d_len = d_off + count;
-    u32 *d_first = (u32 *)(d_len + count), *d_ir = d_first + count, *d_ic = d_ir + ni, *d_raw = d_ic + ni;
+    u64 *d_off = (u64 *)c->crc_buf, *d_len = d_off + count, *d_seg_len = d_len + count;
+    u32 *d_first = (u32 *)(d_seg_len + ns), *d_ir = d_first + count, *d_ic = d_ir + ni, *d_raw = d_ic + ni;
+    u32 *d_seg_first = d_raw + ni, *d_range_seg = d_seg_first + ns, *d_seg_raw = d_range_seg + count;
     if (hipMemcpyAsync(d_off, off, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(d_len, len, 8ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(d_first, first.data(), 4ull * count, hipMemcpyHostToDevice, st) != hipSuccess ||
         (ni && hipMemcpyAsync(d_ir, item_range.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess) ||
-        (ni && hipMemcpyAsync(d_ic, item_chunk.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess))
+        (ni && hipMemcpyAsync(d_ic, item_chunk.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (ns && (hipMemcpyAsync(d_seg_len, seg_len.data(), 8 * ns, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_seg_first, seg_first.data(), 4 * ns, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_range_seg, range_seg.data(), 4ull * count, hipMemcpyHostToDevice, st) !=
+                    hipSuccess)))
         return FQXV_RANS_ERR_DEVICE;
     if (fqxv_launch_crc(d_src, d_off, d_len, d_ir, d_ic, (u32)ni, d_first, count, c->crc_const, d_raw, d_crc,
-                        std::max<u32>(c->dec_max, 256), st) != hipSuccess)
+                        std::max<u32>(c->dec_max, 256), d_seg_len, d_seg_first, (u32)ns, d_range_seg, d_seg_raw,
+                        st) != hipSuccess)
         return FQXV_RANS_ERR_DEVICE;
     // The staging vectors above are host locals: finish before returning.
     return hipStreamSynchronize(st) == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;

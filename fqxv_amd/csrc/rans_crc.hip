@@ -22,6 +22,7 @@ namespace {
 constexpr u32 CRC_CHUNK_LOG = 16;                 // 64 KiB per wave
 constexpr u32 CRC_SEG = 1024;                     // bytes per lane
 constexpr u32 CRC_OPS = 48;                       // operators for 2^0 .. 2^47 bytes
+constexpr u32 SEG_LOG = 12;                       // chunks per segment (2^12 x 64 KiB = 256 MiB)
 #ifndef CRC_GROUP
 #define CRC_GROUP 4                               // 16-byte loads per group
 #endif
@@ -120,29 +121,34 @@ __global__ __launch_bounds__(64) void k_crc_chunks(const u8 *__restrict__ src, c
     }
 }
 
-// One wave per range.  Lane l folds the range's chunks l, l+64, l+128, ...
-// in order (Horner with the operator for 64 chunks = 2^22 bytes, held in
-// registers), shifts its value by the bytes that follow its last chunk, and
-// the wave XORs the lanes; init/xor-out last.
+// One wave per range of `len[r]` bytes made of pieces of 2^piece_log bytes
+// (the last one shorter) whose raw values start at raw[first_item[r]].  Lane
+// l folds pieces l, l+64, l+128, ... in order (Horner with the operator for
+// 64 pieces, held in registers), shifts its value by the bytes that follow
+// its last piece, and the wave XORs the lanes.  `finalize`: apply the
+// init/xor-out terms and write the CRC; else write the range's raw value.
+// Pieces are the 64 KiB chunks, or (for very long ranges, in a second
+// level) segments of 2^SEG_LOG chunks folded by a first call.
 __global__ __launch_bounds__(64) void k_crc_ranges(const u64 *__restrict__ len, const u32 *__restrict__ first_item,
                                                    u32 count, const CrcConst *__restrict__ cc,
-                                                   const u32 *__restrict__ raw, u32 *__restrict__ crc_out)
+                                                   const u32 *__restrict__ raw, u32 *__restrict__ crc_out,
+                                                   u32 piece_log, int finalize)
 {
     const u32 l = lane_id();
     u32 op22[32];
 #pragma unroll
-    for (u32 j = 0; j < 32; j++) op22[j] = cc->op[CRC_CHUNK_LOG + 6][j];
+    for (u32 j = 0; j < 32; j++) op22[j] = cc->op[piece_log + 6][j];
     for (u32 r = blockIdx.x; r < count; r += gridDim.x) {
         const u64 n = len[r];
-        const u64 nch = (n + (1ull << CRC_CHUNK_LOG) - 1) >> CRC_CHUNK_LOG;
+        const u64 nch = (n + (1ull << piece_log) - 1) >> piece_log;
         u32 acc = 0;
         u64 last = 0; // end of this lane's last chunk
         for (u64 k = l; k < nch; k += 64) {
-            const u64 end = (k + 1) << CRC_CHUNK_LOG;
+            const u64 end = (k + 1) << piece_log;
             const u64 cend = end < n ? end : n;
             if (last) {
                 const u64 d = cend - last;
-                if (d == (64ull << CRC_CHUNK_LOG)) {
+                if (d == (64ull << piece_log)) {
                     u32 s2 = 0;
 #pragma unroll
                     for (u32 j = 0; j < 32; j++)
@@ -158,13 +164,14 @@ __global__ __launch_bounds__(64) void k_crc_ranges(const u64 *__restrict__ len, 
         if (last && last < n) acc = zeros_apply(cc, acc, n - last);
 #pragma unroll
         for (u32 h = 32; h >= 1; h >>= 1) acc ^= (u32)__shfl_xor((int)acc, h);
-        if (l == 0) crc_out[r] = ~(zeros_apply(cc, 0xFFFFFFFFu, n) ^ acc);
+        if (l == 0) crc_out[r] = finalize ? ~(zeros_apply(cc, 0xFFFFFFFFu, n) ^ acc) : acc;
     }
 }
 
 } // namespace
 
 size_t fqxv_crc_const_bytes() { return sizeof(CrcConst); }
+uint32_t fqxv_crc_seg_chunks() { return 1u << SEG_LOG; }
 
 // Host: the slicing tables and the zero-byte operators (column j of Z_n is
 // the register after n zero bytes from the single bit j).
@@ -193,15 +200,27 @@ void fqxv_crc_const_fill(void *dst)
         }
 }
 
+// `nseg` == 0: one fold per range over its chunk values.  Else the ranges
+// are first cut into `nseg` segments of at most 2^SEG_LOG chunks
+// (seg_len / seg_first: bytes and first chunk item of each; range r's
+// segments start at range_seg[r]); a first fold gives each segment's raw
+// value, a second folds each range's segments.
 hipError_t fqxv_launch_crc(const u8 *src, const u64 *off, const u64 *len, const u32 *item_range,
                            const u32 *item_chunk, u32 nitems, const u32 *first_item, u32 count, const void *cc,
-                           u32 *raw, u32 *crc_out, u32 grid, hipStream_t st)
+                           u32 *raw, u32 *crc_out, u32 grid, const u64 *seg_len, const u32 *seg_first, u32 nseg,
+                           const u32 *range_seg, u32 *seg_raw, hipStream_t st)
 {
     if (nitems)
         hipLaunchKernelGGL(k_crc_chunks, dim3(grid < nitems ? grid : nitems), dim3(64), 0, st, src, off, len,
                            item_range, item_chunk, nitems, (const CrcConst *)cc, raw);
-    if (count)
+    if (count && nseg == 0)
         hipLaunchKernelGGL(k_crc_ranges, dim3(grid < count ? grid : count), dim3(64), 0, st, len, first_item, count,
-                           (const CrcConst *)cc, raw, crc_out);
+                           (const CrcConst *)cc, raw, crc_out, CRC_CHUNK_LOG, 1);
+    if (count && nseg) {
+        hipLaunchKernelGGL(k_crc_ranges, dim3(grid < nseg ? grid : nseg), dim3(64), 0, st, seg_len, seg_first, nseg,
+                           (const CrcConst *)cc, raw, seg_raw, CRC_CHUNK_LOG, 0);
+        hipLaunchKernelGGL(k_crc_ranges, dim3(grid < count ? grid : count), dim3(64), 0, st, len, range_seg, count,
+                           (const CrcConst *)cc, seg_raw, crc_out, CRC_CHUNK_LOG + SEG_LOG, 1);
+    }
     return hipGetLastError();
 }

@@ -55,3 +55,18 @@ def test_crc32c_batch_large_single_range(coder, oracle):
     got = coder.crc32c_batch(arena, offs, lens)
     torch.cuda.synchronize()
     assert np.array_equal(got.cpu().numpy().view(np.uint32), oracle.crc32c_many(data, offs, lens))
+
+
+def test_crc32c_batch_two_level_fold(coder, oracle):
+    """Ranges longer than 4096 chunks (256 MiB) fold their 256 MiB segments
+    first and the segment values second; a range of exactly 4096 chunks, one
+    of 4097 (a one-byte second segment) and small ranges share the batch."""
+    import torch
+    rng = np.random.default_rng(9)
+    data = rng.integers(0, 256, (260 << 20) + 777, dtype=np.uint8)
+    arena = torch.from_numpy(data).cuda()
+    offs = [0, 5, 1000, 3, 7, 0]
+    lens = [data.size, data.size - 5, 333, (256 << 20) + 1, 256 << 20, 0]
+    got = coder.crc32c_batch(arena, offs, lens)
+    torch.cuda.synchronize()
+    assert np.array_equal(got.cpu().numpy().view(np.uint32), oracle.crc32c_many(data, offs, lens))

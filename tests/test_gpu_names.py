@@ -242,3 +242,15 @@ def test_encode_decode_large_batch(nm, no, coder):
     for names, r, d in zip(lists, recs, dg):
         assert nm.parse_records(r) == names
         assert d == no.names_digest(names)
+
+
+def test_encode_batch_with_a_malformed_records_buffer(nm, no, coder):
+    """A block whose records buffer is truncated fails alone: the others in
+    the same launch still encode (their streams come back packed), and the
+    context serves the next batch normally."""
+    names = [b"r1:1:AC", b"r1:2:AC", b"r1:10:GT"]
+    good = nm.records(names)
+    bad = good[:-1]  # the last name is one byte short
+    with pytest.raises(ValueError, match="block 1"):
+        nm.encode_batch_records([good, bad, good], coder)
+    assert nm.encode_batch_records([good, good], coder) == [no.encode(names)] * 2

@@ -23,8 +23,11 @@ namespace {
 
 constexpr u32 XXH_BLOCK = 1024; // 16 stripes x 64 bytes with the default secret
 constexpr u32 XXH_THREADS = 256;
+#ifndef XXH_WG
+#define XXH_WG 32768
+#endif
 
-__constant__ u8 kSecret[192] = {
+__constant__ __attribute__((aligned(16))) u8 kSecret[192] = {
     0xb8, 0xfe, 0x6c, 0x39, 0x23, 0xa4, 0x4b, 0xbe, 0x7c, 0x01, 0x81, 0x2c, 0xf7, 0x21, 0xad, 0x1c,
     0xde, 0xd4, 0x6d, 0xe9, 0x83, 0x90, 0x97, 0xdb, 0x72, 0x40, 0xa4, 0xa4, 0xb7, 0xb3, 0x67, 0x1f,
     0xcb, 0x79, 0xe6, 0x4e, 0xcc, 0xc0, 0xe5, 0x78, 0x82, 0x5a, 0xd0, 0x7d, 0xcc, 0xff, 0x72, 0x21,
@@ -57,6 +60,8 @@ __device__ __forceinline__ u32 ld32(const u8 *p)
     return (u32)p[0] | ((u32)p[1] << 8) | ((u32)p[2] << 16) | ((u32)p[3] << 24);
 }
 __device__ __forceinline__ u64 sec64(u32 o) { return ld64(kSecret + o); }
+// o a multiple of 8 (the secret is 16-byte aligned): one load.
+__device__ __forceinline__ u64 sec64a(u32 o) { return *(const u64 *)(kSecret + o); }
 
 __device__ __forceinline__ u64 fold64(u64 a, u64 b) { return (a * b) ^ __umul64hi(a, b); }
 __device__ __forceinline__ u64 rotl64(u64 x, u32 r) { return (x << r) | (x >> (64 - r)); }
@@ -125,24 +130,39 @@ __device__ __forceinline__ u64 stripe_lane(u64 w, u64 w_pair, u64 key)
     return (u64)(u32)k * (k >> 32) + w_pair;
 }
 
+// Two stripe words of one lane: words 2m and 2m + 1 of a stripe are each
+// other's pair, so they update accumulator lanes 2m and 2m + 1 without any
+// cross-lane exchange.
+__device__ __forceinline__ void stripe_pair(u64 w0, u64 w1, u64 k0, u64 k1, u64 &a0, u64 &a1)
+{
+    a0 += stripe_lane(w0, w1, k0);
+    a1 += stripe_lane(w1, w0, k1);
+}
+
 // Eight-lane workers: worker w takes the blocks [w * per, (w + 1) * per)
 // of the concatenation of all ranges' blocks (range r holds blocks
 // [first[r], first[r + 1])), finds its first range by binary search and
-// walks forward.  Lane i reads word i of each stripe (the eight lanes read
-// 64 contiguous bytes) and takes word i ^ 1 from its neighbour; the eight
-// lanes of a worker are always active together.
+// walks forward.  Lane j reads bytes 128t + 16j of the block (t < 8): words
+// 2m, 2m + 1 (m = j & 3) of stripe 2t + (j >> 2), and sums them into
+// accumulator lanes 2m, 2m + 1; lanes j and j ^ 4 (the two stripe parities)
+// are added at the end of the block.  The eight lanes of a worker are
+// always active together.
 __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_blocks(const u8 *__restrict__ src, const u64 *__restrict__ off,
                                                              const u64 *__restrict__ first, u32 count, u64 nitems,
                                                              u64 per, u64 *__restrict__ sums)
 {
-    const u32 i = threadIdx.x & 7;
+    const u32 j = threadIdx.x & 7, m = j & 3;
     const u64 w = ((u64)blockIdx.x * XXH_THREADS + threadIdx.x) >> 3;
     u64 it = w * per;
     const u64 end = it + per < nitems ? it + per : nitems;
     if (it >= end) return;
-    u64 key[16];
+    u64 key0[8], key1[8];
 #pragma unroll
-    for (u32 s = 0; s < 16; s++) key[s] = sec64(8 * s + 8 * i);
+    for (u32 t = 0; t < 8; t++) {
+        const u32 so = 8 * (2 * t + (j >> 2)) + 16 * m;
+        key0[t] = sec64a(so);
+        key1[t] = sec64a(so + 8);
+    }
     // Largest r with first[r] <= it (first[count] = nitems > it).
     u32 lo = 0, hi = count;
     while (hi - lo > 1) {
@@ -158,36 +178,52 @@ __global__ __launch_bounds__(XXH_THREADS) void k_xxh3_blocks(const u8 *__restric
             r_end = first[r + 1];
             p = src + off[r] + (it - first[r]) * XXH_BLOCK;
         }
-        u64 acc = 0;
-        if ((((uintptr_t)p) & 7) == 0) {
-            const u64 *q = (const u64 *)p + i;
-            u64 wv[16];
+        u64 a0 = 0, a1 = 0;
+        if ((((uintptr_t)p) & 15) == 0) {
+            const uint4 *q = (const uint4 *)p + j;
+            uint4 v[8];
 #pragma unroll
-            for (u32 s = 0; s < 16; s++) wv[s] = q[8 * s];
+            for (u32 t = 0; t < 8; t++) v[t] = q[8 * t];
 #pragma unroll
-            for (u32 s = 0; s < 16; s++) acc += stripe_lane(wv[s], (u64)__shfl_xor((long long)wv[s], 1, 8), key[s]);
+            for (u32 t = 0; t < 8; t++)
+                stripe_pair((u64)v[t].x | ((u64)v[t].y << 32), (u64)v[t].z | ((u64)v[t].w << 32), key0[t], key1[t],
+                            a0, a1);
         } else {
-            // Word 16s + 2i of the block from aligned dwords: two when the
-            // block is 4-byte aligned, else three and two byte shifts.  Each
-            // dword read holds at least one byte of the block.
+            // 16 bytes from aligned dwords: four when the block is 4-byte
+            // aligned, else five and byte shifts.  Each dword read holds at
+            // least one byte of the block.
             const u32 ph = (u32)((uintptr_t)p & 3);
-            const u32 *A = (const u32 *)(p - ph) + 2 * i;
-            u64 wv[16];
+            const u32 *A = (const u32 *)(p - ph) + 4 * j;
+            u32 d[8][5];
             if (ph == 0) {
 #pragma unroll
-                for (u32 s = 0; s < 16; s++) wv[s] = (u64)A[16 * s] | ((u64)A[16 * s + 1] << 32);
+                for (u32 t = 0; t < 8; t++)
+#pragma unroll
+                    for (u32 e = 0; e < 4; e++) d[t][e] = A[32 * t + e];
+#pragma unroll
+                for (u32 t = 0; t < 8; t++)
+                    stripe_pair((u64)d[t][0] | ((u64)d[t][1] << 32), (u64)d[t][2] | ((u64)d[t][3] << 32), key0[t],
+                                key1[t], a0, a1);
             } else {
 #pragma unroll
-                for (u32 s = 0; s < 16; s++) {
-                    const u32 a0 = A[16 * s], a1 = A[16 * s + 1], a2 = A[16 * s + 2];
-                    wv[s] = (u64)__builtin_amdgcn_alignbyte(a1, a0, ph) |
-                            ((u64)__builtin_amdgcn_alignbyte(a2, a1, ph) << 32);
+                for (u32 t = 0; t < 8; t++)
+#pragma unroll
+                    for (u32 e = 0; e < 5; e++) d[t][e] = A[32 * t + e];
+#pragma unroll
+                for (u32 t = 0; t < 8; t++) {
+                    u32 x[4];
+#pragma unroll
+                    for (u32 e = 0; e < 4; e++) x[e] = __builtin_amdgcn_alignbyte(d[t][e + 1], d[t][e], ph);
+                    stripe_pair((u64)x[0] | ((u64)x[1] << 32), (u64)x[2] | ((u64)x[3] << 32), key0[t], key1[t], a0, a1);
                 }
             }
-#pragma unroll
-            for (u32 s = 0; s < 16; s++) acc += stripe_lane(wv[s], (u64)__shfl_xor((long long)wv[s], 1, 8), key[s]);
         }
-        sums[it * 8 + i] = acc;
+        a0 += (u64)__shfl_xor((long long)a0, 4, 8);
+        a1 += (u64)__shfl_xor((long long)a1, 4, 8);
+        if (j < 4) {
+            sums[it * 8 + 2 * m] = a0;
+            sums[it * 8 + 2 * m + 1] = a1;
+        }
     }
 }
 
@@ -271,8 +307,8 @@ hipError_t fqxv_launch_xxh3(const u8 *src, const u64 *off, const u64 *len, const
                             u64 *sums, u64 *out, hipStream_t st)
 {
     if (nitems) {
-        // About 64 Ki workers (2048 workgroups of 32), at least one block each.
-        const u64 workers_max = 2048ull * (XXH_THREADS / 8);
+        // About XXH_WG workgroups of 32 workers, at least one block each.
+        const u64 workers_max = (u64)XXH_WG * (XXH_THREADS / 8);
         const u64 per = (nitems + workers_max - 1) / workers_max;
         const u64 workers = (nitems + per - 1) / per;
         const u64 blocks = (workers * 8 + XXH_THREADS - 1) / XXH_THREADS;

@@ -38,6 +38,9 @@ hipError_t fqxv_launch_crc(const u8 *, const u64 *, const u64 *, const u32 *, co
                            const void *, u32 *, u32 *, u32, const u64 *, const u32 *, u32, const u32 *, u32 *,
                            hipStream_t);
 uint32_t fqxv_crc_seg_chunks();
+// Waves of the CRC kernels (grid-stride over chunk items and ranges):
+// measured against 2048 and 5120.
+constexpr u32 CRC_GRID = 16384;
 hipError_t fqxv_launch_perm_join(const u8 *, u64, u32 *, hipStream_t);
 hipError_t fqxv_launch_pack_jobs(const u8 *, const fqxv_rans_job *, const u64 *, const i32 *, const u64 *, u32, u8 *,
                                  hipStream_t);
@@ -680,7 +683,7 @@ int fqxv_rans_crc32c_batch(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_
                     hipSuccess)))
         return FQXV_RANS_ERR_DEVICE;
     if (fqxv_launch_crc(d_src, d_off, d_len, d_ir, d_ic, (u32)ni, d_first, count, c->crc_const, d_raw, d_crc,
-                        std::max<u32>(c->dec_max, 256), d_seg_len, d_seg_first, (u32)ns, d_range_seg, d_seg_raw,
+                        CRC_GRID, d_seg_len, d_seg_first, (u32)ns, d_range_seg, d_seg_raw,
                         st) != hipSuccess)
         return FQXV_RANS_ERR_DEVICE;
     // The staging vectors above are host locals: finish before returning.

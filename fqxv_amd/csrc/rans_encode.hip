@@ -342,11 +342,17 @@ __device__ void finish(u8 *dst, u64 hdr, u8 *wend, u64 W, u32 x)
 // aligned dword holding in[0].
 __device__ __forceinline__ u64 bulk_dwords(u64 n) { return n >= 8 ? (n - 8) / 4 + 1 : 0; }
 
-// Histogram of in[0..n) into sh.hist.
+// Histogram of in[0..n) into sh.hist.  Lanes count into HSUB sub-histograms
+// (lane % HSUB, rows of 257 so a byte value sits in a different bank in
+// each) held in sh.fc, which holds nothing yet; with few distinct bytes,
+// fewer lanes then meet on one counter.  The rows are summed at the end.
+constexpr u32 HSUB = 4;
+static_assert(HSUB * 257 <= TAB_CAP, "sub-histograms live in sh.fc");
 __device__ void hist0(EncShared &sh, const u8 *in, u64 n)
 {
     const u32 l = lane_id();
-    for (u32 s = l; s < 256; s += 64) sh.hist[s] = 0;
+    u32 *sub = sh.fc + (l % HSUB) * 257;
+    for (u32 s = l; s < HSUB * 257; s += 64) sh.fc[s] = 0;
     __syncthreads();
     const u64 nd = (n + 3) / 4, nbulk = bulk_dwords(n);
     const u32 ph = (u32)((uintptr_t)in & 3);
@@ -364,13 +370,20 @@ __device__ void hist0(EncShared &sh, const u8 *in, u64 n)
 #pragma unroll
         for (u32 u = 0; u < HB; u++)
 #pragma unroll
-            for (u32 b = 0; b < 4; b++) atomicAdd(&sh.hist[(w[u] >> (8 * b)) & 0xff], 1u);
+            for (u32 b = 0; b < 4; b++) atomicAdd(&sub[(w[u] >> (8 * b)) & 0xff], 1u);
     }
     // The rest (fewer than 64 * HB + 2 dwords), one dword per lane at a time.
     for (u64 t = t0 + l; t < nd; t += 64) {
         const u32 w = load4(in + 4 * t, in + n);
         const u32 nb = (4 * t + 4 <= n) ? 4 : (u32)(n - 4 * t);
-        for (u32 b = 0; b < nb; b++) atomicAdd(&sh.hist[(w >> (8 * b)) & 0xff], 1u);
+        for (u32 b = 0; b < nb; b++) atomicAdd(&sub[(w >> (8 * b)) & 0xff], 1u);
+    }
+    __syncthreads();
+    for (u32 s = l; s < 256; s += 64) {
+        u32 c = 0;
+#pragma unroll
+        for (u32 k = 0; k < HSUB; k++) c += sh.fc[k * 257 + s];
+        sh.hist[s] = c;
     }
     __syncthreads();
 }

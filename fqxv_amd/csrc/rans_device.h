@@ -27,10 +27,11 @@ constexpr u32 HDR0 = 9;                      // [u8 order][u64 n]
 constexpr u32 O0_HDR = 9 + 512 + 4 * NSTATES; // order-0 header through the states
 constexpr u32 BITMAP = 32;
 
-// Per-wave global workspace (bytes).  Order-1 decode keeps, per present
-// context, a 4096-entry u16 slot code table and a 256-entry u32 freq/cum
-// table; order-1 encode keeps 256x256 pair counts and the freq/cum table when
-// the alphabet is too wide for LDS.
+// Per-wave global workspace (bytes).  Order-1 decode keeps its (slot x
+// context) u8 code table in the first 2 MiB (at most 1 MiB, starting at a
+// per-wave offset below 1 MiB) and, for more than 64 contexts, the u32
+// freq/cum table after it; order-1 encode keeps 256x256 pair counts and the
+// freq/cum table when the alphabet is too wide for LDS.
 constexpr u64 WS_CODES = 256ull * 4096 * 2;  // 2 MiB
 constexpr u64 WS_FC = 256ull * 256 * 4;      // 256 KiB
 constexpr u64 WS_PER_WAVE = WS_CODES + WS_FC;

@@ -27,6 +27,7 @@ hipError_t fqxv_launch_decode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32
                                     u64 *, u8 *, u32, hipStream_t);
 hipError_t fqxv_launch_decode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
 hipError_t fqxv_decode_occupancy(int *);
+size_t fqxv_ws_per_wave();
 hipError_t fqxv_launch_encode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32, const u32 *, u32 *, i32 *,
                                     u64 *, u8 *, u32, hipStream_t, int);
 hipError_t fqxv_launch_encode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
@@ -49,7 +50,6 @@ hipError_t fqxv_launch_xxh3(const u8 *, const u64 *, const u64 *, const u64 *, u
 
 namespace {
 
-constexpr u64 WS_PER_WAVE = 256ull * 4096 * 2 + 256ull * 256 * 4; // matches rans_device.h
 constexpr u32 N_COUNTERS = 64;
 
 u64 hdr_bound(u64 n, int order)
@@ -110,7 +110,7 @@ int ensure_ws(fqxv_rans_ctx *c, u64 waves)
         c->ws = nullptr;
         c->ws_waves = 0;
     }
-    if (hipMalloc((void **)&c->ws, waves * WS_PER_WAVE) != hipSuccess) {
+    if (hipMalloc((void **)&c->ws, waves * fqxv_ws_per_wave()) != hipSuccess) {
         c->ws = nullptr;
         return FQXV_RANS_ERR_DEVICE;
     }

@@ -874,6 +874,9 @@ hipError_t fqxv_launch_serve(const u8 *src, u8 *dst, const fqxv_rans_job *ring_j
     return hipGetLastError();
 }
 
+// Bytes of device workspace each persistent decode or encode wave owns.
+size_t fqxv_ws_per_wave() { return WS_PER_WAVE; }
+
 hipError_t fqxv_decode_occupancy(int *blocks_per_cu)
 {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_rans_decode, 64, 0);

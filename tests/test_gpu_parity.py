@@ -279,6 +279,36 @@ def test_order1_wide_alphabets(coder, oracle, k):
     assert out[int(r.dst_off[1]):int(r.dst_off[1]) + len(short)].tobytes() == short
 
 
+def test_order1_widest_tables_on_every_wave(coder, oracle):
+    """Each wave's order-1 code table starts at its own offset in the wave's
+    workspace.  A batch of more widest-table blocks than there are waves --
+    256 contexts (the 1 MiB table of the wide path) and 64 contexts (the
+    256 KiB table of the fast path), alternating -- has every wave, so every
+    offset, build and gather a widest table: decode == input for each."""
+    import torch
+    coder.set_grid(0)
+    waves = coder.max_waves()
+    rng = np.random.default_rng(4242)
+    blocks = []
+    for i in range(waves + 64):
+        k = 256 if i % 2 == 0 else 64
+        b = rng.integers(0, k, 1500 + i % 97).astype(np.uint8)
+        b[:k] = np.arange(k)  # every symbol, so every context, occurs
+        blocks.append(b.tobytes())
+    streams = [oracle.encode(b, 1) for b in blocks]
+    lens = np.array([len(x) for x in streams], np.uint64)
+    offs = np.zeros_like(lens)
+    offs[1:] = np.cumsum(lens)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(streams) + bytes(64), np.uint8).copy()).cuda()
+    r = coder.decode_batch(arena, offs, lens, [len(b) for b in blocks])
+    torch.cuda.synchronize()
+    assert (r.status.cpu().numpy() == 0).all()
+    out = r.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        o = int(r.dst_off[j])
+        assert out[o:o + len(b)].tobytes() == b, j
+
+
 @pytest.mark.parametrize("k", [1, 2, 40, 41, 42, 43, 44, 63, 64])
 @pytest.mark.parametrize("base", [0, 33])
 def test_order1_encode_dense_capacity(coder, oracle, k, base):

@@ -20,9 +20,9 @@
  * of them to `Error::Malformed`.
  *
  * Thread safety: a context serialises its own host-buffer calls with an
- * internal lock.  Batched calls on one context must not overlap on the
- * device (they share the context's work-queue counters and workspace): use
- * one context per concurrently used stream.
+ * internal lock.  Batched and *_one_device calls on one context must not
+ * overlap on the device (they share the context's work-queue counters and
+ * workspace): use one context per concurrently used stream.
  */
 #ifndef FQXV_RANS_HIP_H
 #define FQXV_RANS_HIP_H

@@ -453,10 +453,53 @@ __device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols
     __syncthreads();
 }
 
-// `have_hist`: sh.hist already holds this input's byte histogram (the
-// order-1 pass of encode_best_job follows the order-0 pass, which leaves it).
+// Size bounds of a stream before its walk, for encode_best_job's early
+// choice.  A symbol step maps x to floor(x/f)*4096 + x%f + cum with x >= 16f
+// (x >= 2^16 unrenormalised, x >= 2^20 f >> 16 after one word), so it
+// raises log2(x) by log2(4096/f) + e, log2(15/16) < e <= log2(17/16); a word
+// lowers log2(x) by 16 to 16 + log2(16/15) bits (x >= 2^20 before the
+// shift); each of the 32 states starts at 2^16 and ends in [2^16, 2^32).
+// With S = sum over symbols of log2(4096/f), the W words satisfy
+//   (S - 0.093110 n - 512) / 16.093110 < W <= (S + 0.087463 n) / 16.
+// S is summed in 2^-20 fixed point, each term's log rounded outwards by one
+// unit, so every lane gets the same exact integers.
+struct SizeBounds {
+    u64 lo, hi; // bytes
+};
+constexpr u32 LOG_FRAC = 20;
+__device__ __forceinline__ void acc_log_bits(u32 c, u32 f, u64 &dn, u64 &up)
+{
+    if (c == 0) return;
+    const double L = (12.0 - log2((double)f)) * (double)(1u << LOG_FRAC); // f >= 1 for a counted symbol
+    const u64 lf = (u64)floor(L), lc = (u64)ceil(L);
+    dn += (u64)c * (lf > 0 ? lf - 1 : 0);
+    up += (u64)c * (lc + 1);
+}
+__device__ __forceinline__ SizeBounds size_bounds(u64 dn, u64 up, u64 n, u64 hdr)
+{
+    dn = wave_sum64(dn);
+    up = wave_sum64(up);
+    constexpr u64 ONE = 1ull << LOG_FRAC;
+    const u64 w_max = (up + n * 91751) / (16 * ONE);        // 0.087463 * 2^20 = 91712.4
+    const u64 sub = n * 97728 + 512 * ONE;                   // 0.093110 * 2^20 = 97632.7
+    const double w_lo = dn > sub ? floor((double)(dn - sub) / (16.0932 * (double)ONE)) - 1.0 : 0.0;
+    const u64 w_min = w_lo > 0.0 ? (u64)w_lo : 0;
+    return SizeBounds{hdr + 2 * w_min, hdr + 2 * w_max};
+}
+
+// encode_best_job's early choice: the order-0 bounds in, and after the
+// order-1 tables are built, 1 (order-1 is surely smaller: walk it, skip
+// order-0), 0 (order-0 surely smaller: skip the order-1 walk) or -1 (walk
+// both and compare exactly).
+struct BestGate {
+    SizeBounds b0;
+    int decided;
+};
+
+// `have_hist`: sh.hist already holds this input's byte histogram (computed
+// by encode_best_job before either order).  `gate`: see BestGate.
 __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb_in, u64 &out_len,
-                          u8 *ws, bool have_hist = false)
+                          u8 *ws, bool have_hist = false, BestGate *gate = nullptr)
 {
     const fqxv_rans_job jb = uni_job(jb_in);
     const u32 l = lane_id();
@@ -557,6 +600,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     }
     __syncthreads();
     const u64 hdr = HDR0 + BITMAP + 512ull * K + 4 * NSTATES;
+    u64 bits_dn = 0, bits_up = 0;
     for (u32 d = 0; d < K; d++) {
         const u32 ctx = sh.ctx_of[d];
         u32 c[4], f[4];
@@ -568,6 +612,8 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
                 c[i] = gcounts[ctx * 256 + s];
         }
         normalize_wave(c, f);
+        if (gate)
+            for (int i = 0; i < 4; i++) acc_log_bits(c[i], f[i], bits_dn, bits_up);
         put_freqs(o + HDR0 + BITMAP + 512ull * d, f);
         const u32 tot = f[0] + f[1] + f[2] + f[3];
         u32 cum = wave_scan_incl(tot) - tot;
@@ -588,6 +634,14 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         }
         __syncthreads();
     }
+    if (gate) {
+        const SizeBounds b1 = size_bounds(bits_dn, bits_up, n, hdr);
+        gate->decided = b1.hi < gate->b0.lo ? 1 : (gate->b0.hi < b1.lo ? 0 : -1);
+        if (gate->decided == 0) {
+            out_len = 0;
+            return 0;
+        }
+    }
     if (!dense) __threadfence();
     TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u, asz}
                       : TabRef{gfc, gmg, sh.rowmap, sh.colmap, 2u, 0};
@@ -598,6 +652,9 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     out_len = hdr + 2 * W;
     return 0;
 }
+
+// Blocks from this size on take encode_best_job's early choice.
+constexpr u64 BEST_GATE_MIN = 1ull << 14;
 
 __device__ __forceinline__ u64 encode_bound_dev(u64 n, int order)
 {
@@ -627,12 +684,36 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
     j1.dst_off = jb.dst_off + b0;
     j1.dst_cap = jb.dst_cap - b0;
     u64 l0 = 0, l1 = 0;
-    int err = encode_job(sh, src, dst, j0, l0, ws);
-    if (err) return err;
-    __syncthreads();
-    err = encode_job(sh, src, dst, j1, l1, ws, true);
-    if (err) return err;
-    const bool one = tie == 1 ? l1 <= l0 : l1 < l0;
+    int err;
+    int decided = -1;
+    if (jb.src_len >= BEST_GATE_MIN) {
+        // Byte histogram once; the order-0 size bounds from its normalised
+        // table; order-1 tables next, and a walk only for an order that can
+        // still be the smaller one.
+        hist0(sh, src + jb.src_off, jb.src_len);
+        const u32 l = lane_id();
+        u32 c[4], f[4];
+        for (int i = 0; i < 4; i++) c[i] = sh.hist[4 * l + i];
+        normalize_wave(c, f);
+        u64 dn = 0, up = 0;
+        for (int i = 0; i < 4; i++) acc_log_bits(c[i], f[i], dn, up);
+        BestGate g{size_bounds(dn, up, jb.src_len, O0_HDR), -1};
+        err = encode_job(sh, src, dst, j1, l1, ws, true, &g);
+        if (err) return err;
+        decided = g.decided;
+        if (decided != 1) {
+            __syncthreads();
+            err = encode_job(sh, src, dst, j0, l0, ws, true);
+            if (err) return err;
+        }
+    } else {
+        err = encode_job(sh, src, dst, j0, l0, ws);
+        if (err) return err;
+        __syncthreads();
+        err = encode_job(sh, src, dst, j1, l1, ws, true);
+        if (err) return err;
+    }
+    const bool one = decided >= 0 ? decided == 1 : (tie == 1 ? l1 <= l0 : l1 < l0);
     if (one) {
         __threadfence();
         wave_copy_fwd(dst + jb.dst_off, dst + j1.dst_off, l1);

@@ -279,6 +279,41 @@ def test_order1_wide_alphabets(coder, oracle, k):
     assert out[int(r.dst_off[1]):int(r.dst_off[1]) + len(short)].tobytes() == short
 
 
+def test_encode_best_early_choice_matches_oracle(coder, oracle):
+    """Blocks of 16 KiB and more take the early choice: an order whose size
+    bounds say it is surely the longer stream is not walked.  Cases where
+    the bounds pick order-1 (large Markov qualities), order-0 (small blocks,
+    where the order-1 tables cost more than they save; flat bytes) and
+    neither (i.i.d. bases, names: both walked, exact compare), each under
+    both tie rules: device == oracle, and decoded back."""
+    import torch
+    from fqxv_amd import synth
+    base = [synth.fill(synth.K_MARKOV, 20000, 21).tobytes(), synth.fill(synth.K_MARKOV, 300000, 22).tobytes(),
+            synth.fill(synth.K_DNA, 200000, 23).tobytes(), synth.fill(synth.K_NAMES, 200000, 24).tobytes(),
+            np.random.default_rng(25).integers(0, 256, 70000).astype(np.uint8).tobytes(), b"A" * 50000,
+            synth.fill(synth.K_DNA, 20000, 26).tobytes(), bytes(range(256)) * 300]
+    blocks = base + base
+    ties = np.array([0] * len(base) + [1] * len(base), dtype=np.int32)
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    offs = np.zeros(len(blocks), dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(blocks) + bytes(64), dtype=np.uint8).copy()).cuda()
+    r = coder.encode_best_batch(arena, offs, sizes, ties)
+    torch.cuda.synchronize()
+    assert np.all(r.status.cpu().numpy() == 0)
+    ol = r.out_len.cpu().numpy()
+    host = r.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        got = host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes()
+        assert got == oracle.encode_best(b, int(ties[j])), j
+    d = coder.decode_batch(r.dst, r.dst_off, ol, sizes)
+    torch.cuda.synchronize()
+    assert np.all(d.status.cpu().numpy() == 0)
+    out = d.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        assert out[int(d.dst_off[j]):int(d.dst_off[j]) + len(b)].tobytes() == b, j
+
+
 def test_order1_widest_tables_on_every_wave(coder, oracle):
     """Each wave's order-1 code table starts at its own offset in the wave's
     workspace.  A batch of more widest-table blocks than there are waves --

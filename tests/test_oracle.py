@@ -396,3 +396,44 @@ def test_crc32c_matches_reference_written_checksums(oracle):
     for cut in (0, 1, 999, 5000):
         a, b = x[:cut], x[cut:]
         assert oracle.crc32c_combine(oracle.crc32c(a), oracle.crc32c(b), len(b)) == oracle.crc32c(x)
+
+
+def _size_bound_words(stream: bytes, data: bytes):
+    """(S, n, W) of a stream: S = sum over symbols of log2(4096 / f) with the
+    stream's own normalised tables, W = its renormalisation word count."""
+    n = len(data)
+    a = np.frombuffer(data, np.uint8).astype(np.int64)
+    if stream[0] == 0:
+        f = np.frombuffer(stream[9:9 + 512], "<u2").astype(np.float64)
+        S = float(np.sum(12.0 - np.log2(f[a])))
+        hdr = 9 + 512 + 128
+    else:
+        bm = np.frombuffer(stream[9:41], np.uint8)
+        ctxs = [i for i in range(256) if (bm[i // 8] >> (i % 8)) & 1]
+        k = len(ctxs)
+        tab = np.frombuffer(stream[41:41 + 512 * k], "<u2").reshape(k, 256).astype(np.float64)
+        row = np.full(256, -1, np.int64)
+        row[ctxs] = np.arange(k)
+        prev = np.concatenate([[0], a[:-1]])
+        S = float(np.sum(12.0 - np.log2(tab[row[prev], a])))
+        hdr = 9 + 32 + 512 * k + 128
+    return S, n, (len(stream) - hdr) // 2
+
+
+def test_stream_size_bounds_hold(oracle):
+    """The device's encode-best skips the walk of an order whose stream is
+    surely the longer one, from bounds on its word count W known before the
+    walk: (S - 0.093110 n - 512) / 16.093110 < W <= (S + 0.087463 n) / 16,
+    S = sum of log2(4096 / f) over the symbols.  Checked here on the oracle's
+    streams for skewed, flat, tiny-frequency and single-symbol inputs, both
+    orders, where W is exact."""
+    from fqxv_amd import synth
+    rng = np.random.default_rng(77)
+    cases = [synth.fill(synth.K_MARKOV, 60000, 1).tobytes(), synth.fill(synth.K_DNA, 60000, 2).tobytes(),
+             synth.fill(synth.K_NAMES, 60000, 3).tobytes(), rng.integers(0, 256, 70000).astype(np.uint8).tobytes(),
+             b"A" * 50000, bytes([0] * 40000 + [1]), bytes(rng.choice([0, 1, 2, 255], 65536, p=[.97, .01, .01, .01]).astype(np.uint8)),
+             bytes(rng.integers(0, 2, 100000).astype(np.uint8) * 200)]
+    for j, d in enumerate(cases):
+        for order in (0, 1):
+            S, n, W = _size_bound_words(oracle.encode(d, order), d)
+            assert (S - 0.093110 * n - 512) / 16.093110 < W <= (S + 0.087463 * n) / 16, (j, order, S, n, W)

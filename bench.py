@@ -261,12 +261,18 @@ def bench_encode_best(coder, corpus, arena, oracle, sample):
     r = coder.encode_best_batch(arena, corpus.offs, corpus.sizes, 0, stream=stream)
     torch.cuda.synchronize()
     assert int((r.status != 0).sum()) == 0
-    (a, b), = events(1)
-    a.record(stream)
-    r = coder.encode_best_batch(arena, corpus.offs, corpus.sizes, 0, stream=stream)
-    b.record(stream)
-    torch.cuda.synchronize()
-    t = a.elapsed_time(b) / 1e3
+    times = []
+    for _ in range(2):
+        # The previous result is freed first, so the call reuses its cached
+        # arena instead of allocating a new one inside the timed region.
+        del r
+        (a, b), = events(1)
+        a.record(stream)
+        r = coder.encode_best_batch(arena, corpus.offs, corpus.sizes, 0, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) / 1e3)
+    t = min(times)
     ol = r.out_len.cpu().numpy()
     for j in sample:
         o = int(r.dst_off[j])

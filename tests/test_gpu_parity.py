@@ -314,6 +314,41 @@ def test_encode_best_early_choice_matches_oracle(coder, oracle):
         assert out[int(d.dst_off[j]):int(d.dst_off[j]) + len(b)].tobytes() == b, j
 
 
+def test_encode_best_random_blocks_match_oracle(coder, oracle):
+    """Encode-best over 48 seeded blocks of 16 KiB to 512 KiB (log-uniform),
+    kinds and tie rules drawn at random -- Markov qualities, i.i.d. bases,
+    names, flat bytes, a few-symbol skewed source -- so the early choice
+    meets every outcome at many sizes: device == oracle for every block."""
+    import torch
+    from fqxv_amd import synth
+    rng = np.random.default_rng(31337)
+    blocks, ties = [], []
+    for j in range(48):
+        n = int(np.exp(rng.uniform(np.log(16384), np.log(524288))))
+        kind = int(rng.integers(0, 5))
+        if kind < 3:
+            b = synth.fill((synth.K_MARKOV, synth.K_DNA, synth.K_NAMES)[kind], n, 500 + j).tobytes()
+        elif kind == 3:
+            b = rng.integers(0, 256, n).astype(np.uint8).tobytes()
+        else:
+            b = rng.choice([7, 8, 9, 200], n, p=[0.9, 0.05, 0.04, 0.01]).astype(np.uint8).tobytes()
+        blocks.append(b)
+        ties.append(int(rng.integers(0, 2)))
+    ties = np.array(ties, dtype=np.int32)
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    offs = np.zeros(len(blocks), dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(blocks) + bytes(64), dtype=np.uint8).copy()).cuda()
+    r = coder.encode_best_batch(arena, offs, sizes, ties)
+    torch.cuda.synchronize()
+    assert np.all(r.status.cpu().numpy() == 0)
+    ol = r.out_len.cpu().numpy()
+    host = r.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        got = host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes()
+        assert got == oracle.encode_best(b, int(ties[j])), j
+
+
 def test_order1_widest_tables_on_every_wave(coder, oracle):
     """Each wave's order-1 code table starts at its own offset in the wave's
     workspace.  A batch of more widest-table blocks than there are waves --

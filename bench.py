@@ -126,6 +126,65 @@ def perblock(coder, src, src_off, src_len, dst, dst_off, caps, idx, decode=True,
     return wall, lat
 
 
+def perblock_streams(src, src_off, src_len, dst, dst_off, caps, idx, nstreams=4):
+    """The plain form spread over `nstreams` streams (one context each: a
+    context's single-block calls share its workspace), launches dealt round
+    robin, so the per-block form is not held to one queue."""
+    import torch
+    import fqxv_amd
+    from fqxv_amd import _lib
+    coders = [fqxv_amd.Coder(0) for _ in range(nstreams)]
+    streams = [torch.cuda.Stream() for _ in range(nstreams)]
+    st = torch.zeros(nstreams, dtype=torch.int32, device=src.device)
+    ol = torch.zeros(nstreams, dtype=torch.int64, device=src.device)
+    jobs = [_lib.Job(int(src_off[j]), int(src_len[j]), int(dst_off[j]), int(caps[j]), 2**64 - 1, 0, 0) for j in idx]
+    for k, jb in enumerate(jobs[:nstreams]):
+        coders[k].decode_one_device(src, dst, jb, st[k:k + 1], ol[k:k + 1], stream=streams[k])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, jb in enumerate(jobs):
+        k = i % nstreams
+        coders[k].decode_one_device(src, dst, jb, st[k:k + 1], ol[k:k + 1], stream=streams[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert int((st != 0).sum()) == 0
+    for c in coders:
+        c.close()
+    return wall
+
+
+def bench_percall(coder, oracle):
+    """The drop-in single call (host buffers, PCIe included), as the
+    reference's callers make it: fixture-sized streams (names sub-streams of
+    n <= 1,300 bytes, crates/fqxv-tokenizer/src/lib.rs:98-110) and the 1 MiB
+    binned-quality block, per call mean over repeated calls."""
+    from fqxv_amd import synth
+    res = {}
+    cases = {"names_1000b_o0": (synth.fill(synth.K_NAMES, 1000, 11).tobytes(), 0),
+             "names_1000b_o1": (synth.fill(synth.K_NAMES, 1000, 11).tobytes(), 1),
+             "binned_1mib_o0": (synth.workload("binned_1mib").block(0).tobytes(), 0)}
+    for name, (data, order) in cases.items():
+        reps = 200 if len(data) < 4096 else 10
+        enc = coder.encode(data, order)
+        assert enc == oracle.encode(data, order)
+        assert coder.decode(enc) == data
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            coder.encode(data, order)
+        te = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            coder.decode(enc)
+        td = (time.perf_counter() - t0) / reps
+        res[name] = {"n": len(data), "encode_us": round(te * 1e6, 1), "decode_us": round(td * 1e6, 1),
+                     "decode_mbs": round(len(data) / td / 1e6, 2)}
+    log("[percall] " + ", ".join(f"{k}: enc {v['encode_us']} us / dec {v['decode_us']} us" for k, v in res.items()))
+    return {"form": "host-buffer fqxv_rans_encode / fqxv_rans_decode, one call per stream, PCIe included",
+            "reference_context": "fqxv-rans AVX-512 order-0 decode ~817 MiB/s on one Cascade Lake core "
+                                 "(crates/fqxv-rans/src/lib.rs:176-178; published, other hardware)",
+            **res}
+
+
 def batched_decode_time(coder, enc, comp_len, sizes, dst, reps=3):
     import torch
     stream = torch.cuda.current_stream()
@@ -184,6 +243,11 @@ def bench_c4(args, coder, dev, oracle):
         g.launch(stream)
     torch.cuda.synchronize()
     ev = events(args.steps)
+    # The output region is filled with a sentinel before the timed replays,
+    # so the comparison with the input afterwards proves the timed replays
+    # wrote every byte (the eager decode and the tuner wrote it before).
+    out.fill_(0xEE)
+    g.status.fill_(-1)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for a, b in ev:
@@ -194,7 +258,7 @@ def bench_c4(args, coder, dev, oracle):
     wall = time.perf_counter() - t_start
     kern = [a.elapsed_time(b) / 1e3 for a, b in ev]
     assert int((g.status != 0).sum()) == 0
-    assert torch.equal(out[:n_total], arena[:n_total])
+    assert torch.equal(out[:n_total], arena[:n_total]), "timed replays did not reproduce the input"
     t_step = wall / args.steps
     value = n_total / t_step / 1e9
     t_kern = float(np.mean(kern))
@@ -206,6 +270,21 @@ def bench_c4(args, coder, dev, oracle):
     big1 = [j for j in range(nb) if corpus.orders[j] == 1]
     jbig = max(big1, key=lambda j: corpus.sizes[j]) if big1 else int(np.argmax(corpus.sizes))
     _, lat_big = perblock(coder, enc.dst, enc.dst_off, comp_len, out, dec.dst_off, corpus.sizes, [jbig])
+    # Per-round time of one block alone, per stream kind and order (the
+    # largest block of each class).
+    kind_rounds = {}
+    for kd in sorted(set(int(k) for k in corpus.kinds)):
+        for od in (0, 1):
+            cand = np.nonzero((corpus.kinds == kd) & (corpus.orders == od))[0]
+            if len(cand) == 0:
+                continue
+            j = int(cand[np.argmax(corpus.sizes[cand])])
+            _, lat = perblock(coder, enc.dst, enc.dst_off, comp_len, out, dec.dst_off, corpus.sizes, [j])
+            rounds = (int(corpus.sizes[j]) + 31) // 32
+            kind_rounds[f"{synth.KIND_NAMES[kd]}_o{od}"] = {
+                "block": j, "n": int(corpus.sizes[j]), "ms": round(float(lat[0]), 3),
+                "ns_per_round": round(float(lat[0]) * 1e6 / rounds, 1)}
+    log("[c4] ns/round alone: " + ", ".join(f"{k} {v['ns_per_round']}" for k, v in kind_rounds.items()))
 
     pb = None
     if not args.no_perblock:
@@ -221,6 +300,11 @@ def bench_c4(args, coder, dev, oracle):
               "p99_block_ms": round(float(np.percentile(lat, 99)), 3)}
         log(f"[c4] per-block {pb_gbs:.3f} GB/s, batched speedup {value / pb_gbs:.1f}x, "
             f"p50/p99 {pb['p50_block_ms']}/{pb['p99_block_ms']} ms")
+        wall4 = perblock_streams(enc.dst, enc.dst_off, comp_len, out, dec.dst_off, corpus.sizes, idx)
+        pb["four_streams"] = {"value": round(sb / wall4 / 1e9, 4), "unit": "GB/s",
+                              "form": "one launch per block, dealt round robin over 4 streams (4 contexts)",
+                              "speedup_batched": round(value / (sb / wall4 / 1e9), 1)}
+        log(f"[c4] per-block on 4 streams {sb / wall4 / 1e9:.3f} GB/s")
         torch.cuda.synchronize()
         assert torch.equal(out[:n_total], arena[:n_total])
     xxh = bench_xxh3(coder, arena, corpus.offs, corpus.sizes, args.workload) if args.crc else None
@@ -241,6 +325,7 @@ def bench_c4(args, coder, dev, oracle):
         "corpus": corpus, "value": value, "t_step": t_step, "t_kern": t_kern, "n_total": n_total,
         "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "perblock": pb,
         "tuned": tuned, "eager_ms": eager_ms, "graph_kernel_ms": t_kern * 1e3,
+        "kind_rounds": kind_rounds,
         "largest_o1": {"block": int(jbig), "n": int(corpus.sizes[jbig]), "ms": round(float(lat_big[0]), 3),
                        "rounds": int((int(corpus.sizes[jbig]) + 31) // 32),
                        "ns_per_round": round(float(lat_big[0]) * 1e6 / ((int(corpus.sizes[jbig]) + 31) // 32),
@@ -738,6 +823,7 @@ def main():
             serving = bench_serve(args, coder, dev, st3)
         del st3
     cpu = bench_cpu(args, oracle, c4["cpu_sample"], args.workload) if "cpu" in sections else None
+    percall = bench_percall(coder, oracle) if "configs" in sections else None
 
     n_total, c_total, t_kern = c4["n_total"], c4["c_total"], c4["t_kern"]
     achieved = (n_total + c_total) / t_kern / 1e9
@@ -760,6 +846,7 @@ def main():
                    "blocks": c4["nb"], "uncompressed_bytes": n_total, "compressed_bytes": c_total,
                    "order1_blocks": int(corpus.orders.sum()),
                    "launch": "hipGraph replay of one persistent-kernel launch over all blocks",
+                   "checked": "output filled with a sentinel before the timed replays, equal to the input after",
                    "grid": tuned.grid, "queue_policy": tuned.policy,
                    "generator": "counter-based splitmix64 (fqxv_amd/csrc/synth.c), seeds 1-5",
                    "corpus_xxh3": c4["fingerprint"]},
@@ -770,6 +857,8 @@ def main():
         "perblock": c4["perblock"],
         "batched_eager_ms": c4["eager_ms"],
         "latency_floor": c4["largest_o1"],
+        "ns_per_round_alone": c4["kind_rounds"],
+        "percall": percall,
         "tuner": {"grid": tuned.grid, "policy": tuned.policy, "table": tuned.table},
         "encode": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s", "form": "batched, one persistent launch"},
         "encode_best": c4["encode_best"],

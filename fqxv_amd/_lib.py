@@ -24,6 +24,8 @@ ERR_BOUND = 7
 ERR_DST_TOO_SMALL = 8
 ERR_INVALID_ARG = 9
 ERR_DEVICE = 10
+BACKEND_GFX950 = 0
+BACKEND_GFX950_BATCHED = 1
 NAMES_ERR_FIRST = 32
 
 # Every symbol include/fqxv_rans_hip.h declares.
@@ -42,6 +44,7 @@ EXPORTS = [
     "fqxv_names_decode_batch", "fqxv_names_status", "fqxv_names_records",
     "fqxv_names_batch_destroy", "fqxv_names_digests", "fqxv_rans_xxh3_batch",
     "fqxv_names_encode_batch", "fqxv_names_output", "fqxv_rans_server_create_persistent",
+    "fqxv_rans_xxh3_concat", "fqxv_rans_decode_with", "fqxv_rans_backend_detect",
 ]
 
 
@@ -91,6 +94,8 @@ def load():
         L.fqxv_rans_encode.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz, ctypes.POINTER(sz)]
         L.fqxv_rans_decode.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(sz)]
         L.fqxv_rans_decode_bounded.argtypes = [vp, vp, sz, sz, vp, sz, ctypes.POINTER(sz)]
+        L.fqxv_rans_decode_with.argtypes = [vp, ctypes.c_int, vp, sz, vp, sz, ctypes.POINTER(sz)]
+        L.fqxv_rans_backend_detect.argtypes = []
         L.fqxv_rans_encode_best_bound.restype = sz
         L.fqxv_rans_encode_best_bound.argtypes = [sz]
         for fn in (L.fqxv_rans_decode_batch, L.fqxv_rans_encode_batch, L.fqxv_rans_encode_best_batch):
@@ -132,6 +137,7 @@ def load():
         L.fqxv_names_encode_batch.argtypes = [vp, vp, vp, vp, u32, ctypes.POINTER(vp)]
         L.fqxv_names_output.argtypes = [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64)]
         L.fqxv_rans_xxh3_batch.argtypes = [vp, vp, vp, vp, u32, vp, vp]
+        L.fqxv_rans_xxh3_concat.argtypes = [vp, vp, vp, vp, vp, u32, vp, vp]
         _lib = L
         return L
 

@@ -47,6 +47,7 @@ hipError_t fqxv_launch_pack_jobs(const u8 *, const fqxv_rans_job *, const u64 *,
                                  hipStream_t);
 hipError_t fqxv_launch_xxh3(const u8 *, const u64 *, const u64 *, const u64 *, u32, u64, u64 *, u64 *,
                             hipStream_t);
+hipError_t fqxv_launch_gather_items(const u8 *, const u64 *, const u64 *, const u32 *, u32, u8 *, hipStream_t);
 
 namespace {
 
@@ -84,6 +85,8 @@ struct fqxv_rans_ctx {
     u64 job_cap = 0;
     u8 *xxh_buf = nullptr;                  // per-call xxh3 range/item arrays and block sums
     u64 xxh_cap = 0;
+    u8 *cat_buf = nullptr;                  // concatenated-digest staging and copy items
+    u64 cat_cap = 0;
 };
 
 namespace {
@@ -254,6 +257,7 @@ void fqxv_rans_ctx_destroy(fqxv_rans_ctx *c)
         if (c->crc_buf) (void)hipFree(c->crc_buf);
         if (c->job_buf) (void)hipFree(c->job_buf);
         if (c->xxh_buf) (void)hipFree(c->xxh_buf);
+        if (c->cat_buf) (void)hipFree(c->cat_buf);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -281,8 +285,11 @@ void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_
     std::stable_sort(order_out, order_out + count, [&](u32 a, u32 b) { return sizes[a] > sizes[b]; });
 }
 
-int fqxv_rans_decode_bounded(fqxv_rans_ctx *c, const uint8_t *src, size_t len, size_t max_len, uint8_t *dst,
-                             size_t cap, size_t *out_len)
+// Host-buffer decode through either device form: the single-block launch or
+// the persistent batched kernel given a one-job queue (lib.rs:241-255 forces a
+// backend the same way; the output is identical).
+static int decode_host(fqxv_rans_ctx *c, const uint8_t *src, size_t len, size_t max_len, uint8_t *dst, size_t cap,
+                       size_t *out_len, bool batched)
 {
     if (out_len) *out_len = 0;
     if (!c || (!src && len) || !out_len) return FQXV_RANS_ERR_INVALID_ARG;
@@ -295,13 +302,22 @@ int fqxv_rans_decode_bounded(fqxv_rans_ctx *c, const uint8_t *src, size_t len, s
     if (has_n && n <= (u64)cap && n > 0 && !dst) return FQXV_RANS_ERR_INVALID_ARG;
     int rc;
     if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dcap)) ||
-        (rc = ensure_ws(c, 1)))
+        (rc = ensure_ws(c, 1)) || (batched && (rc = ensure_buf(&c->job_buf, &c->job_cap, sizeof(fqxv_rans_job)))))
         return rc;
     if (len && hipMemcpyAsync(c->d_in, src, len, hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return FQXV_RANS_ERR_DEVICE;
     fqxv_rans_job jb{0, (u64)len, 0, dcap, UINT64_MAX, 0, 0};
-    if (fqxv_launch_decode_one(c->d_in, c->d_out, jb, c->d_status, c->d_out_len, c->ws, c->stream) != hipSuccess)
-        return FQXV_RANS_ERR_DEVICE;
+    hipError_t e;
+    if (batched) {
+        fqxv_rans_job *d_job = (fqxv_rans_job *)c->job_buf;
+        e = hipMemcpyAsync(d_job, &jb, sizeof jb, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = fqxv_launch_decode_batch(c->d_in, c->d_out, d_job, 1, nullptr, next_counter(c), c->d_status,
+                                         c->d_out_len, c->ws, 1, c->stream);
+    } else {
+        e = fqxv_launch_decode_one(c->d_in, c->d_out, jb, c->d_status, c->d_out_len, c->ws, c->stream);
+    }
+    if (e != hipSuccess) return FQXV_RANS_ERR_DEVICE;
     i32 st = 0;
     u64 ol = 0;
     if (hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -313,6 +329,23 @@ int fqxv_rans_decode_bounded(fqxv_rans_ctx *c, const uint8_t *src, size_t len, s
     *out_len = (size_t)ol;
     return FQXV_RANS_OK;
 }
+
+int fqxv_rans_decode_bounded(fqxv_rans_ctx *c, const uint8_t *src, size_t len, size_t max_len, uint8_t *dst,
+                             size_t cap, size_t *out_len)
+{
+    return decode_host(c, src, len, max_len, dst, cap, out_len, false);
+}
+
+int fqxv_rans_decode_with(fqxv_rans_ctx *c, int backend, const uint8_t *src, size_t len, uint8_t *dst, size_t cap,
+                          size_t *out_len)
+{
+    if (out_len) *out_len = 0;
+    if (backend != FQXV_RANS_BACKEND_GFX950 && backend != FQXV_RANS_BACKEND_GFX950_BATCHED)
+        return FQXV_RANS_ERR_INVALID_ARG;
+    return decode_host(c, src, len, SIZE_MAX, dst, cap, out_len, backend == FQXV_RANS_BACKEND_GFX950_BATCHED);
+}
+
+int fqxv_rans_backend_detect(void) { return FQXV_RANS_BACKEND_GFX950; }
 
 int fqxv_rans_decode(fqxv_rans_ctx *c, const uint8_t *src, size_t len, uint8_t *dst, size_t cap, size_t *out_len)
 {
@@ -725,6 +758,51 @@ int fqxv_rans_xxh3_batch(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_t 
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard g(c->device);
     return xxh3_locked(c, d_src, off, len, count, d_out, pick_stream(c, stream));
+}
+
+int fqxv_rans_xxh3_concat(fqxv_rans_ctx *c, const uint8_t *d_src, const uint64_t *seg_off, const uint64_t *seg_len,
+                          const uint32_t *seg_first, uint32_t count, uint64_t *d_out, void *stream)
+{
+    if (!c || (count && (!d_src || !seg_first || !d_out))) return FQXV_RANS_ERR_INVALID_ARG;
+    if (count == 0) return FQXV_RANS_OK;
+    const u32 nseg = seg_first[count];
+    if (nseg && (!seg_off || !seg_len)) return FQXV_RANS_ERR_INVALID_ARG;
+    // Staging layout: each concatenation at a 256-byte aligned start, then
+    // the copy items (pieces of at most 64 KiB).
+    std::vector<u64> off(count), len(count), it_src, it_dst;
+    std::vector<u32> it_len;
+    u64 total = 0;
+    for (u32 r = 0; r < count; r++) {
+        if (seg_first[r] > seg_first[r + 1] || seg_first[r + 1] > nseg) return FQXV_RANS_ERR_INVALID_ARG;
+        off[r] = total;
+        u64 l = 0;
+        for (u32 s = seg_first[r]; s < seg_first[r + 1]; s++) {
+            for (u64 p = 0; p < seg_len[s]; p += 65536) {
+                it_src.push_back(seg_off[s] + p);
+                it_dst.push_back(total + l + p);
+                it_len.push_back((u32)std::min<u64>(65536, seg_len[s] - p));
+            }
+            l += seg_len[s];
+        }
+        len[r] = l;
+        total += (l + 255) & ~255ull;
+    }
+    const u64 ni = it_len.size();
+    if (ni > 0xFFFFFFFFull) return FQXV_RANS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard g(c->device);
+    const u64 items_at = total;
+    int rc = ensure_buf(&c->cat_buf, &c->cat_cap, items_at + 20 * ni + 64);
+    if (rc) return rc;
+    hipStream_t st = pick_stream(c, stream);
+    u64 *d_isrc = (u64 *)(c->cat_buf + items_at), *d_idst = d_isrc + ni;
+    u32 *d_ilen = (u32 *)(d_idst + ni);
+    if (ni && (hipMemcpyAsync(d_isrc, it_src.data(), 8 * ni, hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(d_idst, it_dst.data(), 8 * ni, hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(d_ilen, it_len.data(), 4 * ni, hipMemcpyHostToDevice, st) != hipSuccess ||
+               fqxv_launch_gather_items(d_src, d_isrc, d_idst, d_ilen, (u32)ni, c->cat_buf, st) != hipSuccess))
+        return FQXV_RANS_ERR_DEVICE;
+    return xxh3_locked(c, c->cat_buf, off.data(), len.data(), count, d_out, st);
 }
 
 // Host ranges: packed into the context's input buffer (256-byte aligned

@@ -92,7 +92,28 @@ __global__ __launch_bounds__(64) void k_pack_jobs(const u8 *__restrict__ arena, 
     }
 }
 
+// One wave per item (grid-stride): item i copies len[i] bytes from
+// src + src_off[i] to dst + dst_off[i].  Items are pieces of at most 64 KiB
+// of the segments that a concatenated digest covers; the source arena and the
+// staging buffer do not overlap.
+__global__ __launch_bounds__(64) void k_gather_items(const u8 *__restrict__ src, const u64 *__restrict__ src_off,
+                                                     const u64 *__restrict__ dst_off, const u32 *__restrict__ len,
+                                                     u32 count, u8 *__restrict__ dst)
+{
+    for (u32 i = blockIdx.x; i < count; i += gridDim.x)
+        wave_copy_fwd(dst + uni64(dst_off[i]), src + uni64(src_off[i]), uni(len[i]));
+}
+
 } // namespace
+
+hipError_t fqxv_launch_gather_items(const u8 *src, const u64 *src_off, const u64 *dst_off, const u32 *len, u32 count,
+                                    u8 *dst, hipStream_t st)
+{
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_items, dim3(count < 16384 ? count : 16384), dim3(64), 0, st, src, src_off, dst_off,
+                       len, count, dst);
+    return hipGetLastError();
+}
 
 hipError_t fqxv_launch_pack_jobs(const u8 *arena, const fqxv_rans_job *jobs, const u64 *out_len, const i32 *status,
                                  const u64 *pack_off, u32 count, u8 *packed, hipStream_t st)

@@ -164,9 +164,9 @@ struct Tok {
     i64 value;
 };
 
-// Phase 3 for one stream.  `dec` is the device arena, `job_st`/`job_len`
-// the batch results, `job_off` each job's arena offset.
-int replay(Stream &S, const u8 *dec, const i32 *job_st, const u64 *job_len, const u64 *job_off)
+// Phase 3 for one stream.  `job_ptr`/`job_st`/`job_len` are each device
+// job's decoded bytes, status and length.
+int replay(Stream &S, const u8 *const *job_ptr, const i32 *job_st, const u64 *job_len)
 {
     // Bytes of one decoded sub-stream, or its error, in the reference's
     // order of checks (lib.rs:105-110 / fqxv-rans lib.rs:226-231).
@@ -174,7 +174,7 @@ int replay(Stream &S, const u8 *dec, const i32 *job_st, const u64 *job_len, cons
         if (ev.len >= 9 && ev.declared > bound) return FQXV_RANS_ERR_BOUND;
         const int st = ev.job >= 0 ? job_st[ev.job] : ev.host_st;
         if (st) return st;
-        p = ev.job >= 0 ? dec + job_off[ev.job] : nullptr;
+        p = ev.job >= 0 ? job_ptr[ev.job] : nullptr;
         n = ev.job >= 0 ? job_len[ev.job] : 0;
         return 0;
     };
@@ -432,6 +432,47 @@ int encode_parts(const u8 *rec, u64 size, Parts &P)
     return 0;
 }
 
+// The status a sub-stream gets before its output would be allocated, in the
+// decoder's order of checks (scalar.rs:143-194 with model.rs:160-174; the
+// bound was checked already): 0 when the whole header through the 32 states
+// is present and valid (the stream needs a device job), 1 for n == 0 (an
+// empty output whatever the tag, scalar.rs:147-149), or the error.  Streams
+// that fail here never allocate in the reference, so they get no device
+// output region either.
+int header_status(const u8 *s, u64 len)
+{
+    auto table = [&](u64 at) -> int {
+        u32 total = 0;
+        for (u32 i = 0; i < 256; i++) total += (u32)s[at + 2 * i] | ((u32)s[at + 2 * i + 1] << 8);
+        return total > 4096 ? FQXV_RANS_ERR_FREQ_EXCEEDS : total != 4096 ? FQXV_RANS_ERR_FREQ_SUM : 0;
+    };
+    if (len < 9) return FQXV_RANS_ERR_TRUNCATED;
+    if (peek_n(s) == 0) return 1;
+    const u8 order = s[0];
+    int e;
+    if (order == 0) {
+        if (len < 9 + 512) return FQXV_RANS_ERR_TRUNCATED;
+        if ((e = table(9))) return e;
+        return len < 9 + 512 + 128 ? FQXV_RANS_ERR_TRUNCATED : 0;
+    }
+    if (order != 1) return FQXV_RANS_ERR_UNKNOWN_ORDER;
+    if (len < 9 + 32) return FQXV_RANS_ERR_TRUNCATED;
+    u32 k = 0;
+    for (u32 i = 0; i < 32; i++) k += (u32)__builtin_popcount(s[9 + i]);
+    for (u32 c = 0; c < k; c++) {
+        const u64 at = 41 + 512ull * c;
+        if (len < at + 512) return FQXV_RANS_ERR_TRUNCATED;
+        if ((e = table(at))) return e;
+    }
+    return len < 41 + 512ull * k + 128 ? FQXV_RANS_ERR_TRUNCATED : 0;
+}
+
+// Output bytes of one device launch of the names decoder.  A job declaring
+// more runs in a launch of its own, sized to it; when that region cannot be
+// allocated the job fails like the reference's fallible allocation
+// (model.rs:199-205), and the rest of the batch is unaffected.
+constexpr u64 NAMES_CHUNK = 1ull << 30;
+
 template <class F> void parallel_for(u32 count, F f)
 {
     const u32 hw = std::max(1u, std::thread::hardware_concurrency());
@@ -474,13 +515,13 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
         hi = std::max<u64>(hi, off[i] + len[i]);
     }
     if (count == 0) lo = 0;
-    // Jobs: every sub-stream of >= 9 bytes within its static bound: the flat
-    // streams' max_out; a plane's n_delta never exceeds n_records, and planes
-    // are only read once n_records <= counts length <= max_out has been
-    // checked (lib.rs:565-567), so min(n_records, max_out) bounds them.
+    // Jobs: every sub-stream within its static bound whose header is
+    // complete and valid: the flat streams' bound is max_out; a plane's
+    // n_delta never exceeds n_records, and planes are only read once
+    // n_records <= counts length <= max_out has been checked
+    // (lib.rs:565-567), so min(n_records, max_out) bounds them.  The others
+    // are settled here with the decoder's own error.
     std::vector<fqxv_rans_job> jobs;
-    std::vector<u64> job_off;
-    u64 dst_total = 0;
     for (Stream &S : B->s) {
         bool plane_part = false;
         for (Event &ev : S.ev) {
@@ -495,22 +536,70 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
                 ev.host_st = FQXV_RANS_ERR_BOUND;
                 continue;
             }
+            const int h = header_status(src + ev.off, ev.len);
+            if (h) {
+                ev.host_st = h == 1 ? 0 : h; // n == 0: empty output, no job
+                continue;
+            }
             ev.job = (i64)jobs.size();
-            const u64 cap = (ev.declared + 255) & ~255ull;
-            jobs.push_back(fqxv_rans_job{ev.off - lo, ev.len, dst_total, ev.declared, UINT64_MAX, 0, 0});
-            job_off.push_back(dst_total);
-            dst_total += cap;
+            jobs.push_back(fqxv_rans_job{ev.off, ev.len, 0, ev.declared, UINT64_MAX, 0, 0});
         }
     }
-    // Phase 2: one batched launch over all sub-streams.  The copy back
-    // overwrites the whole arena, so the host buffer is not zero-filled.
-    std::unique_ptr<u8[]> dec(new u8[std::max<u64>(dst_total, 1)]);
+    // Phase 2: the jobs in launches of at most NAMES_CHUNK output bytes
+    // (one launch in the common case).  A launch's staging buffer is kept
+    // while most of it holds decoded output; otherwise the successful
+    // outputs are copied out and it is freed, so corrupt sub-streams that
+    // declare large outputs hold no memory past their launch.
+    std::vector<const u8 *> job_ptr(jobs.size(), nullptr);
     std::vector<i32> job_st(jobs.size());
     std::vector<u64> job_len(jobs.size());
-    if (!jobs.empty()) {
-        const int rc = fqxv_run_jobs_host(ctx, 0, src + lo, hi - lo, jobs.data(), (u32)jobs.size(), dst_total,
-                                          dec.get(), job_st.data(), job_len.data());
+    std::vector<std::unique_ptr<u8[]>> keep;
+    std::vector<std::vector<u8>> copies;
+    for (size_t a = 0; a < jobs.size();) {
+        size_t b = a;
+        u64 total = 0, lo = UINT64_MAX, hi = 0;
+        while (b < jobs.size()) {
+            const u64 cap = (jobs[b].dst_cap + 255) & ~255ull;
+            if (b > a && total + cap > NAMES_CHUNK) break;
+            total += cap;
+            lo = std::min(lo, jobs[b].src_off);
+            hi = std::max(hi, jobs[b].src_off + jobs[b].src_len);
+            b++;
+        }
+        std::vector<fqxv_rans_job> cj(jobs.begin() + a, jobs.begin() + b);
+        u64 at = 0;
+        for (fqxv_rans_job &j : cj) {
+            j.src_off -= lo;
+            j.dst_off = at;
+            at += (j.dst_cap + 255) & ~255ull;
+        }
+        const bool single = b - a == 1;
+        std::unique_ptr<u8[]> stage(new (std::nothrow) u8[std::max<u64>(total, 1)]);
+        int rc = stage ? fqxv_run_jobs_host(ctx, 0, src + lo, hi - lo, cj.data(), (u32)cj.size(), total,
+                                            stage.get(), job_st.data() + a, job_len.data() + a)
+                       : FQXV_RANS_ERR_DEVICE;
+        if (rc == FQXV_RANS_ERR_DEVICE && single && cj[0].dst_cap > NAMES_CHUNK) {
+            job_st[a] = FQXV_RANS_ERR_OUTPUT_ALLOC;
+            job_len[a] = 0;
+            a = b;
+            continue;
+        }
         if (rc) return rc;
+        u64 good = 0;
+        for (size_t j = a; j < b; j++)
+            if (job_st[j] == 0) good += job_len[j];
+        if (2 * good >= total) {
+            for (size_t j = a; j < b; j++) job_ptr[j] = stage.get() + cj[j - a].dst_off;
+            keep.push_back(std::move(stage));
+        } else {
+            for (size_t j = a; j < b; j++) {
+                if (job_st[j] != 0 || job_len[j] == 0) continue;
+                const u8 *p = stage.get() + cj[j - a].dst_off;
+                copies.emplace_back(p, p + job_len[j]);
+                job_ptr[j] = copies.back().data();
+            }
+        }
+        a = b;
     }
     // Phase 3, threads across streams.
     const u32 hw = std::max(1u, std::thread::hardware_concurrency());
@@ -520,7 +609,7 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
         for (u32 i; (i = next.fetch_add(1)) < count;) {
             Stream &S = B->s[i];
             try {
-                S.status = replay(S, dec.get(), job_st.data(), job_len.data(), job_off.data());
+                S.status = replay(S, job_ptr.data(), job_st.data(), job_len.data());
             } catch (const std::bad_alloc &) {
                 S.status = FQXV_NAMES_ERR_ALLOC; // the reference's fallible reserves (lib.rs:150-152, 244-245)
             }

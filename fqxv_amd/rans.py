@@ -49,7 +49,9 @@ class Backend(enum.Enum):
 
     @staticmethod
     def detect() -> "Backend":
-        return Backend.Gfx950
+        """lib.rs:80-96: the form plain ``decode`` uses (fqxv_rans_backend_detect)."""
+        code = _lib.load().fqxv_rans_backend_detect()
+        return Backend.Gfx950Batched if code == _lib.BACKEND_GFX950_BATCHED else Backend.Gfx950
 
 
 class Error(Exception):
@@ -164,6 +166,18 @@ class Coder:
 
     def decode(self, src) -> bytes:
         return self.decode_bounded(src, (1 << 64) - 1)
+
+    def decode_with(self, src, backend: Backend) -> bytes:
+        """lib.rs:241-255: decode pinned to one device form."""
+        code = {Backend.Gfx950: _lib.BACKEND_GFX950, Backend.Gfx950Batched: _lib.BACKEND_GFX950_BATCHED}[backend]
+        b = _as_bytes(src)
+        n = peek_output_len(b)
+        cap = n if (n is not None and n <= ALLOC_LIMIT) else 0
+        dst = ctypes.create_string_buffer(max(cap, 1))
+        out = ctypes.c_size_t(0)
+        sbuf = ctypes.create_string_buffer(b, len(b)) if b else None
+        _raise(self._L.fqxv_rans_decode_with(self._h, code, sbuf, len(b), dst, cap, ctypes.byref(out)))
+        return dst.raw[: out.value]
 
     # ------------------------------------------------ device-resident forms
     @staticmethod
@@ -338,6 +352,25 @@ class Coder:
                                             len(offs), ctypes.c_void_p(out.data_ptr()), self._stream(stream)))
         return out[: len(offs)]
 
+    def xxh3_concat(self, src, segments, stream=None):
+        """XXH3-64 of concatenated byte ranges of a device arena:
+        ``segments[i]`` is a list of (offset, length) pairs whose bytes, in
+        order, form string i (e.g. ``[n_reads u64][lens u32...][bytes]``,
+        the sequence/quality digest layout of block.rs:164-196).  Returns a
+        device int64 tensor holding the u64 digests' bits."""
+        import torch
+        first = np.zeros(len(segments) + 1, dtype=np.uint32)
+        first[1:] = np.cumsum([len(s) for s in segments])
+        flat = [p for s in segments for p in s]
+        offs = np.ascontiguousarray([p[0] for p in flat] or [0], dtype=np.uint64)
+        lens = np.ascontiguousarray([p[1] for p in flat] or [0], dtype=np.uint64)
+        out = torch.zeros(max(len(segments), 1), dtype=torch.int64, device=src.device)
+        _raise(self._L.fqxv_rans_xxh3_concat(self._h, ctypes.c_void_p(src.data_ptr()),
+                                             ctypes.c_void_p(offs.ctypes.data), ctypes.c_void_p(lens.ctypes.data),
+                                             ctypes.c_void_p(first.ctypes.data), len(segments),
+                                             ctypes.c_void_p(out.data_ptr()), self._stream(stream)))
+        return out[: len(segments)]
+
     def set_grid(self, waves: int) -> None:
         """Persistent-grid size of later batched calls (0 = occupancy max)."""
         _raise(self._L.fqxv_rans_ctx_set_grid(self._h, int(waves)))
@@ -418,20 +451,6 @@ def decode_bounded(src, max_len: int) -> bytes:
 
 
 def decode_with(src, backend: Backend) -> bytes:
-    """lib.rs:241-255: force a device form; output is identical."""
-    if backend == Backend.Gfx950:
-        return default_coder().decode(src)
-    import torch
-    b = _as_bytes(src)
-    n = peek_output_len(b)
-    if n is None:
-        raise Malformed(_lib.strerror(_lib.ERR_TRUNCATED), _lib.ERR_TRUNCATED)
-    c = default_coder()
-    dev = torch.device("cuda", c.device)
-    s = torch.frombuffer(bytearray(b + b"\0" * 8), dtype=torch.uint8).to(dev)
-    cap = n if n <= ALLOC_LIMIT else 0
-    r = c.decode_batch(s, [0], [len(b)], [cap])
-    torch.cuda.synchronize(dev)
-    st = int(r.status[0])
-    _raise(st)
-    return r.dst[: int(r.out_len[0])].cpu().numpy().tobytes()
+    """lib.rs:241-255: decode through the named device form
+    (``fqxv_rans_decode_with``); output is identical."""
+    return default_coder().decode_with(src, backend)

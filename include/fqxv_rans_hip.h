@@ -126,6 +126,23 @@ int fqxv_rans_decode(fqxv_rans_ctx *ctx, const uint8_t *src, size_t len, uint8_t
 int fqxv_rans_decode_bounded(fqxv_rans_ctx *ctx, const uint8_t *src, size_t len, size_t max_len,
                              uint8_t *dst, size_t cap, size_t *out_len);
 
+/* Device forms a decode can be pinned to: the single-block launch, or the
+ * persistent batched kernel fed a one-job queue.  Both produce identical
+ * output.  They stand where lib.rs:67-97 `Backend` (Scalar/Sse42/Avx2/
+ * Avx512) stands. */
+#define FQXV_RANS_BACKEND_GFX950 0
+#define FQXV_RANS_BACKEND_GFX950_BATCHED 1
+
+/* lib.rs:241-255 `decode_with(src, backend)`: decode through the named
+ * device form.  Same arguments and errors as fqxv_rans_decode; an unknown
+ * backend is FQXV_RANS_ERR_INVALID_ARG. */
+int fqxv_rans_decode_with(fqxv_rans_ctx *ctx, int backend, const uint8_t *src, size_t len, uint8_t *dst,
+                          size_t cap, size_t *out_len);
+
+/* lib.rs:80-96 `Backend::detect()`: the form fqxv_rans_decode uses
+ * (FQXV_RANS_BACKEND_GFX950). */
+int fqxv_rans_backend_detect(void);
+
 /* Batched device-resident encode/decode: one persistent-kernel launch codes
  * all `count` jobs.  All pointers are device pointers (d_jobs included).
  * `d_order` (optional, may be NULL) is a permutation of 0..count-1 giving the
@@ -304,6 +321,22 @@ int fqxv_names_digests(fqxv_rans_ctx *ctx, const fqxv_names_batch *b, uint64_t *
  * Work runs on `stream`; the call returns when the digests are written. */
 int fqxv_rans_xxh3_batch(fqxv_rans_ctx *ctx, const uint8_t *d_src, const uint64_t *off, const uint64_t *len,
                          uint32_t count, uint64_t *d_out, void *stream);
+
+/* XXH3-64 of `count` byte strings, each the concatenation of several ranges
+ * of the device arena `d_src`: digest i covers segments
+ * seg_first[i] .. seg_first[i + 1] - 1, where segment s is
+ * [seg_off[s], seg_off[s] + seg_len[s]), in that order.  The sequence and
+ * quality digests of crates/fqxv/src/container/block.rs:164-196 (checked at
+ * block.rs:1097-1109) hash `[n_reads u64][lens u32...][bytes]`, whose parts
+ * live apart: with the header and the lengths in the arena, a block's digest
+ * is three segments.  Host arrays (seg_first holds count + 1 entries);
+ * d_out[i] (device) receives digest i.  The segments are staged
+ * back to back on the device (the context owns the staging buffer) and
+ * hashed as one range each.  Work runs on `stream`; the call returns when
+ * the digests are written. */
+int fqxv_rans_xxh3_concat(fqxv_rans_ctx *ctx, const uint8_t *d_src, const uint64_t *seg_off,
+                          const uint64_t *seg_len, const uint32_t *seg_first, uint32_t count, uint64_t *d_out,
+                          void *stream);
 
 /* Host helper: job order, largest `sizes[i]` first (ties by index). */
 void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_out);

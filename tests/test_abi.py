@@ -93,3 +93,16 @@ def test_vectorised_encode_bounds_match_the_library(L):
     assert [int(v) for v in got] == [int(L.fqxv_rans_encode_bound(n, o)) for n, o in zip(ns, mixed)]
     got = Coder.encode_best_bounds(ns)
     assert [int(v) for v in got] == [int(L.fqxv_rans_encode_best_bound(n)) for n in ns]
+
+
+C_DRIVER = os.path.join(ROOT, "tests", "c_driver", "_build", "abi_driver")
+
+
+def test_c_driver_links_against_the_library():
+    """The plain-C driver (tests/c_driver, built by __graft_entry__.build())
+    links libfqxv_rans_hip.so and the HIP runtime and resolves the host-only
+    entry points without a device."""
+    import subprocess
+    assert os.path.exists(C_DRIVER), "tests/c_driver is not built (run __graft_entry__.build())"
+    r = subprocess.run([C_DRIVER, "--probe"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "probe ok" in r.stdout, r.stdout + r.stderr

@@ -464,3 +464,37 @@ def test_order1_wide_malformed_same_error(coder, oracle):
             assert coder.decode(c) == out_o
         checked += 1
     assert checked > 30
+
+
+def test_decode_with_every_backend(coder, oracle, fixture_streams):
+    """lib.rs:241-255 `decode_with`: each device form (the single-block launch
+    and the batched kernel fed a one-job queue) decodes byte-identically to the
+    oracle, on the edge cases and the reference-written fixture streams, and
+    fails malformed streams with the oracle's error; Backend::detect names the
+    form plain decode uses."""
+    import fqxv_amd
+    from fqxv_amd import Backend
+    assert Backend.detect() == Backend.Gfx950
+    streams = [oracle.encode(d, o) for d in _cases() for o in (0, 1)] + [s for _, s in fixture_streams]
+    for s in streams:
+        ref = oracle.decode(s)
+        for be in (Backend.Gfx950, Backend.Gfx950Batched):
+            assert coder.decode_with(s, be) == ref, (len(s), be)
+    assert fqxv_amd.decode_with(streams[3], Backend.Gfx950Batched) == oracle.decode(streams[3])
+    r = np.random.default_rng(78)
+    checked = 0
+    for it in range(120):
+        data = r.integers(0, 30, r.integers(0, 2000)).astype(np.uint8).tobytes()
+        enc = bytearray(oracle.encode(data, it & 1))
+        for _ in range(r.integers(1, 6)):
+            if enc:
+                enc[r.integers(0, len(enc))] = r.integers(0, 256)
+        b = bytes(enc)
+        n = oracle.peek_len(b)
+        if n is not None and n > max(len(data), 1) * 4:
+            continue
+        rc, out, _ = oracle.decode_ex(b)
+        for be in (Backend.Gfx950, Backend.Gfx950Batched):
+            assert _err_code(lambda: coder.decode_with(b, be)) == rc, (it, be)
+        checked += 1
+    assert checked > 60

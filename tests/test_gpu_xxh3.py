@@ -57,3 +57,48 @@ def test_names_digests_of_failed_and_large_streams(coder, oracle):
     assert dg[0] == no.names_digest(big)
     assert dg[1] == 0 and isinstance(recs[1], names.Error)
     assert dg[2] == no.names_digest([])
+
+
+def test_concat_digest_block_layout(coder):
+    """fqxv_rans_xxh3_concat on the sequence/quality digest layout of
+    crates/fqxv/src/container/block.rs:164-196: XXH3-64 over
+    [n_reads u64][lens u32...][bytes], the parts placed apart in the arena
+    (header, lengths and bytes at unaligned offsets), checked against the
+    xxhash package on the same bytes; plus empty segments, an empty string,
+    segments longer than one 64 KiB staging piece, and many strings in one
+    call."""
+    import torch
+    rng = np.random.default_rng(31)
+    pieces, cases = [], []
+    pos = 0
+
+    def put(b):
+        nonlocal pos
+        pos += int(rng.integers(0, 13))  # misalign
+        o = pos
+        pieces.append((o, b))
+        pos += len(b)
+        return o
+
+    for n_reads in (0, 1, 7, 1000, 25000):
+        lens = rng.integers(0, 301, n_reads).astype("<u4")
+        seq = rng.integers(0, 256, int(lens.sum())).astype(np.uint8).tobytes()
+        hdr = np.array([n_reads], dtype="<u8").tobytes()
+        segs = [(put(hdr), 8), (put(lens.tobytes()), 4 * n_reads), (put(seq), len(seq))]
+        cases.append((segs, hdr + lens.tobytes() + seq))
+    big = rng.integers(0, 256, (1 << 20) + 3).astype(np.uint8).tobytes()
+    cases.append(([(put(big), len(big)), (put(b""), 0), (put(b"xyz"), 3)], big + b"xyz"))
+    cases.append(([], b""))
+    one = rng.integers(0, 256, 5000).astype(np.uint8).tobytes()
+    cases.append(([(put(one), len(one))], one))
+    arena = np.zeros(pos + 64, dtype=np.uint8)
+    for o, b in pieces:
+        arena[o:o + len(b)] = np.frombuffer(b, dtype=np.uint8)
+    d = torch.from_numpy(arena).cuda()
+    out = coder.xxh3_concat(d, [c[0] for c in cases]).cpu().numpy().view(np.uint64)
+    for (segs, ref), h in zip(cases, out):
+        assert int(h) == xxhash.xxh3_64_intdigest(ref), [s[1] for s in segs]
+    # The same strings as single ranges through the plain batch entry point.
+    single = [c for c in cases if len(c[0]) == 1]
+    got = coder.xxh3_batch(d, [c[0][0][0] for c in single], [c[0][0][1] for c in single]).cpu().numpy()
+    assert [int(x) for x in got.view(np.uint64)] == [xxhash.xxh3_64_intdigest(c[1]) for c in single]

@@ -457,11 +457,12 @@ int fqxv_run_jobs_host(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_
     return FQXV_RANS_OK;
 }
 
-int fqxv_encode_jobs_host_packed(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_len,
-                                 const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total,
-                                 uint8_t **h_packed, uint64_t *h_pack_off, int32_t *h_status, uint64_t *h_out_len)
+int fqxv_jobs_host_packed(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_t src_len,
+                          const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total, uint8_t **h_packed,
+                          uint64_t *h_pack_off, int32_t *h_status, uint64_t *h_out_len)
 {
-    if (!c || !h_packed || (src_len && !h_src) || (count && (!h_jobs || !h_pack_off || !h_status || !h_out_len)))
+    if (!c || !h_packed || (mode != 0 && mode != 2) || (src_len && !h_src) ||
+        (count && (!h_jobs || !h_pack_off || !h_status || !h_out_len)))
         return FQXV_RANS_ERR_INVALID_ARG;
     *h_packed = nullptr;
     std::vector<u64> cost(count);
@@ -474,12 +475,18 @@ int fqxv_encode_jobs_host_packed(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_
     if (count) fqxv_rans_lpt_order(cost.data(), count, order.data());
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard g(c->device);
-    const u32 grid = std::min<u32>(std::max<u32>(count, 1), c->enc_waves);
+    const u32 grid = std::min<u32>(std::max<u32>(count, 1), mode ? c->enc_waves : c->dec_waves);
     const u64 jb = sizeof(fqxv_rans_job) * (u64)count, ob = 4ull * count, sb = 4ull * count, lb = 8ull * count;
     int rc;
     if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, src_len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dst_total)) ||
         (rc = ensure_buf(&c->job_buf, &c->job_cap, 2 * lb + jb + ob + sb)) || (rc = ensure_ws(c, grid)))
         return rc;
+    auto launch = [&](u32 *ctr, i32 *d_st, u64 *d_len, fqxv_rans_job *d_jobs, u32 *d_order, hipStream_t st) {
+        return mode ? fqxv_launch_encode_batch(c->d_in, c->d_out, d_jobs, count, d_order, ctr, d_st, d_len, c->ws,
+                                               grid, st, 1)
+                    : fqxv_launch_decode_batch(c->d_in, c->d_out, d_jobs, count, d_order, ctr, d_st, d_len, c->ws,
+                                               grid, st);
+    };
     u64 *d_len = (u64 *)c->job_buf;
     u64 *d_poff = d_len + count;
     fqxv_rans_job *d_jobs = (fqxv_rans_job *)(d_poff + count);
@@ -490,8 +497,7 @@ int fqxv_encode_jobs_host_packed(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_
         if ((src_len && hipMemcpyAsync(c->d_in, h_src, src_len, hipMemcpyHostToDevice, st) != hipSuccess) ||
             hipMemcpyAsync(d_jobs, h_jobs, jb, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(d_order, order.data(), ob, hipMemcpyHostToDevice, st) != hipSuccess ||
-            fqxv_launch_encode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len, c->ws,
-                                     grid, st, 1) != hipSuccess ||
+            launch(next_counter(c), d_st, d_len, d_jobs, d_order, st) != hipSuccess ||
             hipMemcpyAsync(h_status, d_st, sb, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(h_out_len, d_len, lb, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)

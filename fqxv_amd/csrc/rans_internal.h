@@ -16,13 +16,14 @@ int fqxv_run_jobs_host(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_
                        const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total, uint8_t *h_dst,
                        int32_t *h_status, uint64_t *h_out_len);
 
-// Mode 2 of fqxv_run_jobs_host with the outputs packed: the job regions
-// stay on the device; only the successful jobs' output bytes come back, job
-// i's at (*h_packed)[h_pack_off[i]].  *h_packed is allocated with malloc
-// (the caller frees it); nothing is allocated on an error.  Synchronous.
-int fqxv_encode_jobs_host_packed(fqxv_rans_ctx *c, const uint8_t *h_src, uint64_t src_len,
-                                 const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total,
-                                 uint8_t **h_packed, uint64_t *h_pack_off, int32_t *h_status, uint64_t *h_out_len);
+// fqxv_run_jobs_host (mode 0 decode, mode 2 encode-best) with the outputs
+// packed: the job regions stay on the device; only the successful jobs'
+// output bytes come back, job i's at (*h_packed)[h_pack_off[i]], so a job
+// that fails holds no host memory.  *h_packed is allocated with malloc (the
+// caller frees it); nothing is allocated on an error.  Synchronous.
+int fqxv_jobs_host_packed(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_t src_len,
+                          const fqxv_rans_job *h_jobs, uint32_t count, uint64_t dst_total, uint8_t **h_packed,
+                          uint64_t *h_pack_off, int32_t *h_status, uint64_t *h_out_len);
 
 // xxh3-64 of `count` host byte ranges on the device (uploaded, hashed,
 // digests copied to out).  Synchronous.

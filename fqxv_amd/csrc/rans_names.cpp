@@ -164,33 +164,53 @@ struct Tok {
     i64 value;
 };
 
-// Phase 3 for one stream.  `job_ptr`/`job_st`/`job_len` are each device
-// job's decoded bytes, status and length.
-int replay(Stream &S, const u8 *const *job_ptr, const i32 *job_st, const u64 *job_len)
-{
-    // Bytes of one decoded sub-stream, or its error, in the reference's
-    // order of checks (lib.rs:105-110 / fqxv-rans lib.rs:226-231).
-    auto bytes_of = [&](const Event &ev, u64 bound, const u8 *&p, u64 &n) -> int {
-        if (ev.len >= 9 && ev.declared > bound) return FQXV_RANS_ERR_BOUND;
-        const int st = ev.job >= 0 ? job_st[ev.job] : ev.host_st;
-        if (st) return st;
-        p = ev.job >= 0 ? job_ptr[ev.job] : nullptr;
-        n = ev.job >= 0 ? job_len[ev.job] : 0;
-        return 0;
-    };
+// Decoded sub-streams of the device phases, indexed by job.
+struct JobOut {
+    std::vector<const u8 *> ptr;
+    std::vector<i32> st;
+    std::vector<u64> len;
+};
+
+// A stream's state between its flat sub-streams and its delta planes.
+struct Mid {
     const u8 *fp[NFLAT] = {};
     u64 fn[NFLAT] = {};
+    std::vector<std::vector<u8>> ops;
+    u64 n_delta[NCOL] = {};
+    size_t e = 0; // first event of the planes part
+};
+
+// Bytes of one decoded sub-stream, or its error, in the reference's order of
+// checks (lib.rs:105-110 / fqxv-rans lib.rs:226-231).
+int bytes_of(const Event &ev, u64 bound, const JobOut &J, const u8 *&p, u64 &n)
+{
+    if (ev.len >= 9 && ev.declared > bound) return FQXV_RANS_ERR_BOUND;
+    const int st = ev.job >= 0 ? J.st[ev.job] : ev.host_st;
+    if (st) return st;
+    p = ev.job >= 0 ? J.ptr[ev.job] : nullptr;
+    n = ev.job >= 0 ? J.len[ev.job] : 0;
+    return 0;
+}
+
+// Phase 3a for one stream: the flat sub-streams, the record-count check and
+// the op run-length expansion (lib.rs:547-588, 202-245).
+int replay_flat(const Stream &S, Mid &M, const JobOut &J)
+{
     size_t e = 0;
     const std::vector<Event> &ev = S.ev;
+    const u8 **fp = M.fp;
+    u64 *fn = M.fn;
     for (u32 k = 0; k < NFLAT; k++, e++) {
         if (e >= ev.size()) return FQXV_NAMES_ERR_TRUNCATED; // unreachable: the walk ends in EV_ERR
         if (ev[e].kind == EV_ERR) return ev[e].err;
-        int rc = bytes_of(ev[e], S.max_out, fp[k], fn[k]);
+        int rc = bytes_of(ev[e], S.max_out, J, fp[k], fn[k]);
         if (rc) return rc;
         if (k == 0 && S.n_records > fn[0]) return FQXV_NAMES_ERR_RECORD_COUNT; // lib.rs:565-567
     }
+    M.e = e;
     // Op run-length expansion (lib.rs:202-245).
-    std::vector<std::vector<u8>> ops(NCOL);
+    std::vector<std::vector<u8>> &ops = M.ops;
+    ops.assign(NCOL, {});
     {
         Cur cc(fp[5], fn[5]), cl(fp[7], fn[7]);
         const u8 *syms = fp[6];
@@ -212,8 +232,19 @@ int replay(Stream &S, const u8 *const *job_ptr, const i32 *job_st, const u64 *jo
             }
         }
     }
-    u64 n_delta[NCOL];
-    for (u32 c = 0; c < NCOL; c++) n_delta[c] = (u64)std::count(ops[c].begin(), ops[c].end(), OP_DELTA);
+    for (u32 c = 0; c < NCOL; c++) M.n_delta[c] = (u64)std::count(ops[c].begin(), ops[c].end(), OP_DELTA);
+    return 0;
+}
+
+// Phase 3b for one stream: the delta planes and the rebuild.
+int replay_planes(Stream &S, Mid &M, const JobOut &J)
+{
+    size_t e = M.e;
+    const std::vector<Event> &ev = S.ev;
+    const u8 *const *fp = M.fp;
+    const u64 *fn = M.fn;
+    const std::vector<std::vector<u8>> &ops = M.ops;
+    const u64 *n_delta = M.n_delta;
     // Delta byte planes (lib.rs:137-168).
     std::vector<std::vector<u64>> deltas(NCOL);
     for (u32 c = 0; c < NCOL; c++) {
@@ -228,7 +259,7 @@ int replay(Stream &S, const u8 *const *job_ptr, const i32 *job_st, const u64 *jo
             if (ev[e].kind == EV_ERR) return ev[e].err;
             const u8 *p;
             u64 n;
-            int rc = bytes_of(ev[e], n_delta[c], p, n);
+            int rc = bytes_of(ev[e], n_delta[c], J, p, n);
             if (rc) return rc;
             if (n != n_delta[c]) return FQXV_NAMES_ERR_PLANE_LEN;
             const u32 sh = (u32)((8 * pl) & 63); // release-mode `<<` of lib.rs:163
@@ -435,7 +466,7 @@ int encode_parts(const u8 *rec, u64 size, Parts &P)
 // The status a sub-stream gets before its output would be allocated, in the
 // decoder's order of checks (scalar.rs:143-194 with model.rs:160-174; the
 // bound was checked already): 0 when the whole header through the 32 states
-// is present and valid (the stream needs a device job), 1 for n == 0 (an
+// is present and valid (the stream needs a device job), -1 for n == 0 (an
 // empty output whatever the tag, scalar.rs:147-149), or the error.  Streams
 // that fail here never allocate in the reference, so they get no device
 // output region either.
@@ -447,7 +478,7 @@ int header_status(const u8 *s, u64 len)
         return total > 4096 ? FQXV_RANS_ERR_FREQ_EXCEEDS : total != 4096 ? FQXV_RANS_ERR_FREQ_SUM : 0;
     };
     if (len < 9) return FQXV_RANS_ERR_TRUNCATED;
-    if (peek_n(s) == 0) return 1;
+    if (peek_n(s) == 0) return -1;
     const u8 order = s[0];
     int e;
     if (order == 0) {
@@ -490,6 +521,55 @@ template <class F> void parallel_for(u32 count, F f)
     for (auto &t : th) t.join();
 }
 
+// Runs jobs[first..) on the device, in launches of at most NAMES_CHUNK
+// output bytes (a larger job alone, in a region of its own size; when that
+// region cannot be allocated, the job fails like the reference's fallible
+// allocation, model.rs:199-205).  Only the successful jobs' bytes come back
+// to the host (packed), so a corrupt sub-stream declaring a large output
+// holds no host memory.  Fills J for those jobs; J keeps the buffers.
+int run_phase(fqxv_rans_ctx *ctx, const u8 *src, const std::vector<fqxv_rans_job> &jobs, size_t first, JobOut &J,
+              std::vector<std::unique_ptr<u8, void (*)(void *)>> &keep)
+{
+    J.ptr.resize(jobs.size(), nullptr);
+    J.st.resize(jobs.size(), 0);
+    J.len.resize(jobs.size(), 0);
+    for (size_t a = first; a < jobs.size();) {
+        size_t b = a;
+        u64 total = 0, lo = UINT64_MAX, hi = 0;
+        while (b < jobs.size()) {
+            const u64 cap = (jobs[b].dst_cap + 255) & ~255ull;
+            if (b > a && total + cap > NAMES_CHUNK) break;
+            total += cap;
+            lo = std::min(lo, jobs[b].src_off);
+            hi = std::max(hi, jobs[b].src_off + jobs[b].src_len);
+            b++;
+        }
+        std::vector<fqxv_rans_job> cj(jobs.begin() + a, jobs.begin() + b);
+        u64 at = 0;
+        for (fqxv_rans_job &j : cj) {
+            j.src_off -= lo;
+            j.dst_off = at;
+            at += (j.dst_cap + 255) & ~255ull;
+        }
+        std::vector<u64> poff(cj.size());
+        u8 *packed = nullptr;
+        const int rc = fqxv_jobs_host_packed(ctx, 0, src + lo, hi - lo, cj.data(), (u32)cj.size(), total, &packed,
+                                             poff.data(), J.st.data() + a, J.len.data() + a);
+        if (rc == FQXV_RANS_ERR_DEVICE && b - a == 1 && cj[0].dst_cap > NAMES_CHUNK) {
+            J.st[a] = FQXV_RANS_ERR_OUTPUT_ALLOC;
+            J.len[a] = 0;
+            a = b;
+            continue;
+        }
+        if (rc) return rc;
+        keep.emplace_back(packed, free);
+        for (size_t j = a; j < b; j++)
+            if (J.st[j] == 0) J.ptr[j] = packed + poff[j - a];
+        a = b;
+    }
+    return 0;
+}
+
 } // namespace
 
 struct fqxv_names_batch {
@@ -508,126 +588,92 @@ static int names_decode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
     std::unique_ptr<fqxv_names_batch> B(new fqxv_names_batch());
     B->s.resize(count);
     // Phase 1.
-    u64 lo = UINT64_MAX, hi = 0;
-    for (u32 i = 0; i < count; i++) {
-        frame(src, off[i], len[i], B->s[i]);
-        lo = std::min<u64>(lo, off[i]);
-        hi = std::max<u64>(hi, off[i] + len[i]);
-    }
-    if (count == 0) lo = 0;
-    // Jobs: every sub-stream within its static bound whose header is
-    // complete and valid: the flat streams' bound is max_out; a plane's
-    // n_delta never exceeds n_records, and planes are only read once
-    // n_records <= counts length <= max_out has been checked
-    // (lib.rs:565-567), so min(n_records, max_out) bounds them.  The others
-    // are settled here with the decoder's own error.
+    for (u32 i = 0; i < count; i++) frame(src, off[i], len[i], B->s[i]);
+    // Phase 2a: the flat sub-streams of every stream, each bounded by its
+    // stream's max_out.  A sub-stream whose header is incomplete or invalid
+    // is settled here with the decoder's own error (it never allocates in the
+    // reference either), so only complete headers get a device job.
     std::vector<fqxv_rans_job> jobs;
+    JobOut J;
+    auto add_job = [&](Event &ev, u64 bound) {
+        if (ev.len < 9) {
+            ev.host_st = FQXV_RANS_ERR_TRUNCATED; // scalar.rs: tag + u64 n need 9 bytes
+            return;
+        }
+        if (ev.declared > bound) {
+            ev.host_st = FQXV_RANS_ERR_BOUND;
+            return;
+        }
+        const int h = header_status(src + ev.off, ev.len);
+        if (h) {
+            ev.host_st = h < 0 ? 0 : h; // n == 0: empty output, no job
+            return;
+        }
+        ev.job = (i64)jobs.size();
+        jobs.push_back(fqxv_rans_job{ev.off, ev.len, 0, ev.declared, UINT64_MAX, 0, 0});
+    };
     for (Stream &S : B->s) {
-        bool plane_part = false;
+        u32 k = 0;
         for (Event &ev : S.ev) {
-            if (ev.kind == EV_WIDTH) plane_part = true;
-            if (ev.kind != EV_SUB) continue;
-            const u64 bound = plane_part ? std::min(S.n_records, S.max_out) : S.max_out;
-            if (ev.len < 9) {
-                ev.host_st = FQXV_RANS_ERR_TRUNCATED; // scalar.rs: tag + u64 n need 9 bytes
-                continue;
-            }
-            if (ev.declared > bound) {
-                ev.host_st = FQXV_RANS_ERR_BOUND;
-                continue;
-            }
-            const int h = header_status(src + ev.off, ev.len);
-            if (h) {
-                ev.host_st = h == 1 ? 0 : h; // n == 0: empty output, no job
-                continue;
-            }
-            ev.job = (i64)jobs.size();
-            jobs.push_back(fqxv_rans_job{ev.off, ev.len, 0, ev.declared, UINT64_MAX, 0, 0});
+            if (ev.kind != EV_SUB || k == NFLAT) break;
+            add_job(ev, S.max_out);
+            k++;
         }
     }
-    // Phase 2: the jobs in launches of at most NAMES_CHUNK output bytes
-    // (one launch in the common case).  A launch's staging buffer is kept
-    // while most of it holds decoded output; otherwise the successful
-    // outputs are copied out and it is freed, so corrupt sub-streams that
-    // declare large outputs hold no memory past their launch.
-    std::vector<const u8 *> job_ptr(jobs.size(), nullptr);
-    std::vector<i32> job_st(jobs.size());
-    std::vector<u64> job_len(jobs.size());
-    std::vector<std::unique_ptr<u8[]>> keep;
-    std::vector<std::vector<u8>> copies;
-    for (size_t a = 0; a < jobs.size();) {
-        size_t b = a;
-        u64 total = 0, lo = UINT64_MAX, hi = 0;
-        while (b < jobs.size()) {
-            const u64 cap = (jobs[b].dst_cap + 255) & ~255ull;
-            if (b > a && total + cap > NAMES_CHUNK) break;
-            total += cap;
-            lo = std::min(lo, jobs[b].src_off);
-            hi = std::max(hi, jobs[b].src_off + jobs[b].src_len);
-            b++;
+    std::vector<std::unique_ptr<u8, void (*)(void *)>> keep;
+    int rc = run_phase(ctx, src, jobs, 0, J, keep);
+    if (rc) return rc;
+    // Phase 3a, threads across streams; then the planes of the streams that
+    // got through, each bounded by its column's delta count (lib.rs:137-168),
+    // so no plane region is larger than the output it must match.
+    std::vector<Mid> mid(count);
+    parallel_for(count, [&](u32 i) {
+        Stream &S = B->s[i];
+        try {
+            S.status = replay_flat(S, mid[i], J);
+        } catch (const std::bad_alloc &) {
+            S.status = FQXV_NAMES_ERR_ALLOC; // the reference's fallible reserves (lib.rs:244-245)
         }
-        std::vector<fqxv_rans_job> cj(jobs.begin() + a, jobs.begin() + b);
-        u64 at = 0;
-        for (fqxv_rans_job &j : cj) {
-            j.src_off -= lo;
-            j.dst_off = at;
-            at += (j.dst_cap + 255) & ~255ull;
-        }
-        const bool single = b - a == 1;
-        std::unique_ptr<u8[]> stage(new (std::nothrow) u8[std::max<u64>(total, 1)]);
-        int rc = stage ? fqxv_run_jobs_host(ctx, 0, src + lo, hi - lo, cj.data(), (u32)cj.size(), total,
-                                            stage.get(), job_st.data() + a, job_len.data() + a)
-                       : FQXV_RANS_ERR_DEVICE;
-        if (rc == FQXV_RANS_ERR_DEVICE && single && cj[0].dst_cap > NAMES_CHUNK) {
-            job_st[a] = FQXV_RANS_ERR_OUTPUT_ALLOC;
-            job_len[a] = 0;
-            a = b;
-            continue;
-        }
-        if (rc) return rc;
-        u64 good = 0;
-        for (size_t j = a; j < b; j++)
-            if (job_st[j] == 0) good += job_len[j];
-        if (2 * good >= total) {
-            for (size_t j = a; j < b; j++) job_ptr[j] = stage.get() + cj[j - a].dst_off;
-            keep.push_back(std::move(stage));
-        } else {
-            for (size_t j = a; j < b; j++) {
-                if (job_st[j] != 0 || job_len[j] == 0) continue;
-                const u8 *p = stage.get() + cj[j - a].dst_off;
-                copies.emplace_back(p, p + job_len[j]);
-                job_ptr[j] = copies.back().data();
+    });
+    const size_t first_plane = jobs.size();
+    for (u32 i = 0; i < count; i++) {
+        Stream &S = B->s[i];
+        if (S.status) continue;
+        const Mid &M = mid[i];
+        u32 c = 0;
+        u64 left = 0; // planes left in column c - 1
+        for (size_t e = M.e; e < S.ev.size(); e++) {
+            Event &ev = S.ev[e];
+            if (ev.kind == EV_ERR) break;
+            if (ev.kind == EV_WIDTH) {
+                c++;
+                left = ev.width;
+                continue;
             }
+            if (c == 0 || left == 0) break; // unreachable: planes follow their width
+            add_job(ev, M.n_delta[c - 1]);
+            left--;
         }
-        a = b;
     }
-    // Phase 3, threads across streams.
-    const u32 hw = std::max(1u, std::thread::hardware_concurrency());
-    const u32 nthr = std::min<u32>({count, hw, 16u});
-    std::atomic<u32> next{0};
-    auto work = [&]() {
-        for (u32 i; (i = next.fetch_add(1)) < count;) {
-            Stream &S = B->s[i];
+    if ((rc = run_phase(ctx, src, jobs, first_plane, J, keep))) return rc;
+    // Phase 3b.
+    parallel_for(count, [&](u32 i) {
+        Stream &S = B->s[i];
+        if (!S.status) {
             try {
-                S.status = replay(S, job_ptr.data(), job_st.data(), job_len.data());
+                S.status = replay_planes(S, mid[i], J);
             } catch (const std::bad_alloc &) {
                 S.status = FQXV_NAMES_ERR_ALLOC; // the reference's fallible reserves (lib.rs:150-152, 244-245)
             }
-            if (S.status) {
-                S.rec.clear();
-                S.n_names = 0;
-            }
-            S.ev.clear();
-            S.ev.shrink_to_fit();
         }
-    };
-    if (nthr <= 1) {
-        work();
-    } else {
-        std::vector<std::thread> th;
-        for (u32 t = 0; t < nthr; t++) th.emplace_back(work);
-        for (auto &t : th) t.join();
-    }
+        if (S.status) {
+            S.rec.clear();
+            S.n_names = 0;
+        }
+        mid[i] = Mid();
+        S.ev.clear();
+        S.ev.shrink_to_fit();
+    });
     *out = B.release();
     return FQXV_RANS_OK;
 }
@@ -692,7 +738,7 @@ static int names_encode_batch(fqxv_rans_ctx *ctx, const uint8_t *src, const uint
     std::vector<i32> st(jobs.size());
     std::vector<u64> ol(jobs.size()), poff(jobs.size());
     u8 *packed = nullptr;
-    const int rc = fqxv_encode_jobs_host_packed(ctx, raw.get(), src_total, jobs.data(), (u32)jobs.size(), dst_total,
+    const int rc = fqxv_jobs_host_packed(ctx, 2, raw.get(), src_total, jobs.data(), (u32)jobs.size(), dst_total,
                                                 &packed, poff.data(), st.data(), ol.data());
     if (rc) return rc;
     std::unique_ptr<u8, decltype(&free)> enc(packed, &free);

@@ -37,6 +37,7 @@
 hipError_t fqxv_launch_serve(const uint8_t *, uint8_t *, const fqxv_rans_job *, fqxv_serve_res *, uint32_t,
                              const fqxv_serve_ctl *, fqxv_serve_dev *, uint8_t *, uint64_t, uint32_t, hipStream_t);
 hipError_t fqxv_decode_occupancy(int *);
+size_t fqxv_ws_per_wave();
 
 typedef uint8_t u8;
 typedef uint32_t u32;
@@ -511,7 +512,7 @@ int fqxv_rans_server_create_persistent(int device, const uint8_t *d_archive, con
          hipHostGetDevicePointer((void **)&s->dp_jobs, s->h_jobs, 0) == hipSuccess &&
          hipHostGetDevicePointer((void **)&s->dp_res, s->h_res, 0) == hipSuccess &&
          hipMalloc((void **)&s->d_dev, sizeof(fqxv_serve_dev)) == hipSuccess &&
-         hipMalloc((void **)&s->d_ws, (size_t)s->waves * (256ull * 4096 * 2 + 256ull * 256 * 4)) == hipSuccess &&
+         hipMalloc((void **)&s->d_ws, (size_t)s->waves * fqxv_ws_per_wave()) == hipSuccess &&
          hipStreamCreateWithFlags(&s->pstream, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&s->pdone, hipEventDisableTiming) == hipSuccess;
     if (ok) {

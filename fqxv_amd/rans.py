@@ -111,6 +111,10 @@ class BatchResult:
     dst_off: np.ndarray  # per-job output offsets (host)
     out_len: object      # torch int64 device tensor
     status: object       # torch int32 device tensor
+    # The device job and order arrays the launch reads: held as long as the
+    # result, so the caching allocator cannot hand them out while the
+    # launch (on whatever stream it was given) may still read them.
+    inputs: tuple = ()
 
 
 class Coder:
@@ -209,6 +213,12 @@ class Coder:
         ordt = None
         if order_np is not None:
             ordt = torch.from_numpy(np.ascontiguousarray(order_np, dtype=np.uint32).view(np.int32)).to(dev)
+        if stream is not None:
+            # Allocated on the current stream, read on `stream`.
+            s = torch.cuda.ExternalStream(stream.cuda_stream) if not isinstance(stream, torch.cuda.Stream) else stream
+            for t in (jobs, status, out_len, ordt):
+                if t is not None:
+                    t.record_stream(s)
         rc = fn(self._h, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
                 ctypes.c_void_p(jobs.data_ptr()), k,
                 ctypes.c_void_p(ordt.data_ptr()) if ordt is not None else None,
@@ -260,8 +270,7 @@ class Coder:
         # Largest estimated encode time first (order-1 bytes cost more).
         order_np = tune.queue_order(self, "lpt-cost", src_len, orders, "encode") if lpt else None
         status, out_len, keep = self._run_batch(self._L.fqxv_rans_encode_batch, src, dst, jobs, order_np, stream)
-        self._keep = keep
-        return BatchResult(dst, dst_off, out_len, status)
+        return BatchResult(dst, dst_off, out_len, status, keep)
 
     def encode_best_bound(self, n: int) -> int:
         return int(self._L.fqxv_rans_encode_best_bound(int(n)))
@@ -285,8 +294,7 @@ class Coder:
         order_np = self.lpt_order(src_len) if lpt else None
         status, out_len, keep = self._run_batch(self._L.fqxv_rans_encode_best_batch, src, dst, jobs, order_np,
                                                 stream)
-        self._keep = keep
-        return BatchResult(dst, dst_off, out_len, status)
+        return BatchResult(dst, dst_off, out_len, status, keep)
 
     def decode_batch(self, src, src_off, src_len, out_len, max_len=None, lpt: bool = True,
                      stream=None, dst=None, dst_off=None) -> BatchResult:
@@ -306,8 +314,7 @@ class Coder:
         jobs = self.jobs_array(src_off, src_len, dst_off, caps, max_len=max_len)
         order_np = self.lpt_order(caps) if lpt else None
         status, ol, keep = self._run_batch(self._L.fqxv_rans_decode_batch, src, dst, jobs, order_np, stream)
-        self._keep = keep
-        return BatchResult(dst, dst_off, ol, status)
+        return BatchResult(dst, dst_off, ol, status, keep)
 
     def _one(self, fn, src, dst, job: _lib.Job, status, out_len, stream):
         rc = fn(self._h, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),

@@ -437,3 +437,64 @@ def test_stream_size_bounds_hold(oracle):
         for order in (0, 1):
             S, n, W = _size_bound_words(oracle.encode(d, order), d)
             assert (S - 0.093110 * n - 512) / 16.093110 < W <= (S + 0.087463 * n) / 16, (j, order, S, n, W)
+
+
+def _device_size_bounds(stream: bytes, data: bytes):
+    """The device's integer form of those bounds (acc_log_bits / size_bounds
+    in fqxv_amd/csrc/rans_encode.hip): each symbol's log2(4096 / f) in
+    2^-20 fixed point, rounded outwards by one unit, summed per symbol count,
+    then the constants 91751 and 97728 and the divisor 16.0932.  Returns
+    (lo, hi) in stream bytes."""
+    import math
+    ONE = 1 << 20
+    n = len(data)
+    a = np.frombuffer(data, np.uint8).astype(np.int64)
+    if stream[0] == 0:
+        tabs = np.frombuffer(stream[9:9 + 512], "<u2").astype(np.int64)[None, :]
+        rows = np.zeros(n, np.int64)
+        hdr = 9 + 512 + 128
+    else:
+        bm = np.frombuffer(stream[9:41], np.uint8)
+        ctxs = [i for i in range(256) if (bm[i // 8] >> (i % 8)) & 1]
+        tabs = np.frombuffer(stream[41:41 + 512 * len(ctxs)], "<u2").reshape(len(ctxs), 256).astype(np.int64)
+        row = np.full(256, -1, np.int64)
+        row[ctxs] = np.arange(len(ctxs))
+        rows = row[np.concatenate([[0], a[:-1]])]
+        hdr = 9 + 32 + 512 * len(ctxs) + 128
+    counts = np.zeros(tabs.shape, np.int64)
+    np.add.at(counts, (rows, a), 1)
+    dn = up = 0
+    for (r, s), c in np.ndenumerate(counts):
+        if c == 0:
+            continue
+        L = (12.0 - math.log2(float(tabs[r, s]))) * ONE
+        lf, lc = math.floor(L), math.ceil(L)
+        dn += int(c) * (lf - 1 if lf > 0 else 0)
+        up += int(c) * (lc + 1)
+    w_max = (up + n * 91751) // (16 * ONE)
+    sub = n * 97728 + 512 * ONE
+    w_lo = math.floor((dn - sub) / (16.0932 * ONE)) - 1.0 if dn > sub else 0.0
+    w_min = int(w_lo) if w_lo > 0 else 0
+    return hdr + 2 * w_min, hdr + 2 * w_max
+
+
+def test_device_integer_size_bounds_bracket_true_sizes(oracle):
+    """The integer constants the device uses bracket each order's true stream
+    size on the cases above (so a later change to them is caught here), and
+    they are at least as wide as the exact real-valued bounds."""
+    from fqxv_amd import synth
+    rng = np.random.default_rng(78)
+    cases = [synth.fill(synth.K_MARKOV, 60000, 1).tobytes(), synth.fill(synth.K_DNA, 60000, 2).tobytes(),
+             synth.fill(synth.K_NAMES, 60000, 3).tobytes(), rng.integers(0, 256, 70000).astype(np.uint8).tobytes(),
+             b"A" * 50000, bytes([0] * 40000 + [1]),
+             bytes(rng.choice([0, 1, 2, 255], 65536, p=[.97, .01, .01, .01]).astype(np.uint8)),
+             synth.fill(synth.K_MARKOV, 1 << 20, 4).tobytes()]
+    for j, d in enumerate(cases):
+        for order in (0, 1):
+            s = oracle.encode(d, order)
+            lo, hi = _device_size_bounds(s, d)
+            assert lo <= len(s) <= hi, (j, order, lo, len(s), hi)
+            S, n, W = _size_bound_words(s, d)
+            hdr = len(s) - 2 * W
+            assert lo <= hdr + 2 * max(0.0, (S - 0.093110 * n - 512) / 16.093110) + 2
+            assert hi >= hdr + 2 * int((S + 0.087463 * n) / 16)

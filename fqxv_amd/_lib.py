@@ -24,6 +24,7 @@ ERR_BOUND = 7
 ERR_DST_TOO_SMALL = 8
 ERR_INVALID_ARG = 9
 ERR_DEVICE = 10
+ERR_SCRATCH_LIMIT = 11
 BACKEND_GFX950 = 0
 BACKEND_GFX950_BATCHED = 1
 NAMES_ERR_FIRST = 32
@@ -45,6 +46,7 @@ EXPORTS = [
     "fqxv_names_batch_destroy", "fqxv_names_digests", "fqxv_rans_xxh3_batch",
     "fqxv_names_encode_batch", "fqxv_names_output", "fqxv_rans_server_create_persistent",
     "fqxv_rans_xxh3_concat", "fqxv_rans_decode_with", "fqxv_rans_backend_detect",
+    "fqxv_rans_decode_batch_noalloc", "fqxv_rans_noalloc_max_waves",
 ]
 
 
@@ -96,6 +98,9 @@ def load():
         L.fqxv_rans_decode_bounded.argtypes = [vp, vp, sz, sz, vp, sz, ctypes.POINTER(sz)]
         L.fqxv_rans_decode_with.argtypes = [vp, ctypes.c_int, vp, sz, vp, sz, ctypes.POINTER(sz)]
         L.fqxv_rans_backend_detect.argtypes = []
+        L.fqxv_rans_decode_batch_noalloc.argtypes = [vp, vp, vp, ctypes.c_uint32, vp, vp, vp, ctypes.c_uint32, vp]
+        L.fqxv_rans_noalloc_max_waves.argtypes = []
+        L.fqxv_rans_noalloc_max_waves.restype = ctypes.c_uint32
         L.fqxv_rans_encode_best_bound.restype = sz
         L.fqxv_rans_encode_best_bound.argtypes = [sz]
         for fn in (L.fqxv_rans_decode_batch, L.fqxv_rans_encode_batch, L.fqxv_rans_encode_best_batch):

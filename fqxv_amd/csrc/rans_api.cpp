@@ -172,6 +172,7 @@ const char *fqxv_rans_strerror(int code)
     case FQXV_RANS_ERR_DST_TOO_SMALL: return "destination buffer too small";
     case FQXV_RANS_ERR_INVALID_ARG: return "invalid argument";
     case FQXV_RANS_ERR_DEVICE: return "HIP device error";
+    case FQXV_RANS_ERR_SCRATCH_LIMIT: return "request exceeds the fixed scratch of the no-allocation call";
     case FQXV_NAMES_ERR_TRUNCATED: return "truncated";
     case FQXV_NAMES_ERR_VARINT: return "varint too long";
     case FQXV_NAMES_ERR_VERSION: return "unsupported version";

@@ -30,6 +30,8 @@
 #include "rans_device.h"
 #include "rans_serve.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr u32 RING_CHUNK_WORDS = 256;                // 512 bytes per chunk
@@ -880,4 +882,60 @@ size_t fqxv_ws_per_wave() { return WS_PER_WAVE; }
 hipError_t fqxv_decode_occupancy(int *blocks_per_cu)
 {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_rans_decode, 64, 0);
+}
+
+// ------------------------------------------ batched decode without allocation
+// For callers that cannot allocate (code under graph capture, tight serving
+// loops): the batched decode with its work-queue counter and per-wave
+// workspace taken from a fixed device array instead of memory the library
+// allocates.  The array holds the counter (first 256 bytes) and
+// NOALLOC_WAVES per-wave workspaces: 256 + 32 x 2.25 MiB = 72 MiB, held by
+// every GPU the library runs on until the process ends.  All calls on one
+// GPU share that GPU's copy, so calls that overlap on one GPU race (they
+// must be ordered, e.g. issued on one stream); the counter is reset by a
+// kernel at the start of every call, nothing else needs to start at zero.
+constexpr u32 NOALLOC_WAVES = 32;
+constexpr u64 NOALLOC_SCRATCH = 256 + (u64)NOALLOC_WAVES * WS_PER_WAVE;
+static __device__ __align__(256) u8 scratch[NOALLOC_SCRATCH];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+u8 *noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of this GPU's copy of `scratch`, looked up once per device
+// with that device current (the host-side name is only a stand-in).
+hipError_t noalloc_scratch(int dev, u8 **out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        noalloc_ptr[dev] = (u8 *)p;
+    }
+    *out = noalloc_ptr[dev];
+    return hipSuccess;
+}
+} // namespace
+
+extern "C" uint32_t fqxv_rans_noalloc_max_waves(void) { return NOALLOC_WAVES; }
+
+extern "C" int fqxv_rans_decode_batch_noalloc(const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
+                                              uint32_t count, const uint32_t *d_order, int32_t *d_status,
+                                              uint64_t *d_out_len, uint32_t waves, void *stream)
+{
+    if (waves > NOALLOC_WAVES) return FQXV_RANS_ERR_SCRATCH_LIMIT;
+    if (count == 0) return FQXV_RANS_OK;
+    if (!d_src || !d_dst || !d_jobs || !d_status || !d_out_len) return FQXV_RANS_ERR_INVALID_ARG;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+    u8 *base = nullptr;
+    if (noalloc_scratch(dev, &base) != hipSuccess) return FQXV_RANS_ERR_DEVICE;
+    const u32 w = waves ? waves : NOALLOC_WAVES;
+    const u32 grid = count < w ? count : w;
+    const hipError_t e = fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, (u32 *)base, d_status,
+                                                  d_out_len, base + 256, grid, (hipStream_t)stream);
+    return e == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
 }

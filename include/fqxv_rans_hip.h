@@ -45,6 +45,7 @@ extern "C" {
 #define FQXV_RANS_ERR_DST_TOO_SMALL 8    /* encode: destination smaller than fqxv_rans_encode_bound   */
 #define FQXV_RANS_ERR_INVALID_ARG 9      /* bad order / NULL pointer / bad job                         */
 #define FQXV_RANS_ERR_DEVICE 10          /* HIP runtime failure (allocation, launch, copy)             */
+#define FQXV_RANS_ERR_SCRATCH_LIMIT 11   /* request exceeds the fixed scratch of the no-allocation call */
 
 /* Names-stream errors: the reference's Error::Malformed strings of
  * crates/fqxv-tokenizer/src/lib.rs (a sub-stream's rANS failure keeps its
@@ -343,6 +344,22 @@ void fqxv_rans_lpt_order(const uint64_t *sizes, uint32_t count, uint32_t *order_
 
 /* Persistent-grid size used by the batched calls (workgroups of one wave). */
 int fqxv_rans_max_waves(fqxv_rans_ctx *ctx, uint32_t *waves);
+
+/* fqxv_rans_decode_batch for callers that cannot allocate device memory
+ * (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  Same jobs, order, status and
+ * out_len contract, on the current HIP device, asynchronous on `stream`; no
+ * context.  The work-queue counter and the per-wave workspaces come from a
+ * fixed device array of the library (about 72 MiB, one copy per GPU, held
+ * until the process ends) instead of allocations: the persistent grid is at
+ * most fqxv_rans_noalloc_max_waves() waves (`waves` 0 = that maximum; more
+ * is FQXV_RANS_ERR_SCRATCH_LIMIT), any number of jobs.  Calls that overlap
+ * on one GPU share its array and race: order them (one stream).  The call
+ * allocates nothing, on the device or the host. */
+int fqxv_rans_decode_batch_noalloc(const uint8_t *d_src, uint8_t *d_dst, const fqxv_rans_job *d_jobs,
+                                   uint32_t count, const uint32_t *d_order, int32_t *d_status,
+                                   uint64_t *d_out_len, uint32_t waves, void *stream);
+uint32_t fqxv_rans_noalloc_max_waves(void);
 
 #ifdef __cplusplus
 }

@@ -19,14 +19,16 @@
 //   * decoded bytes are staged in LDS and written 1 KiB (32 rounds) at a
 //     time with one 16-byte store per lane;
 //   * order-1: symbol i's context is symbol i-1 (scalar.rs:203), a serial
-//     chain through the round.  Each present context (K <= 64) owns a lane;
-//     each round every lane fetches, for all 32 slots, the code of the symbol
-//     its context would decode there (u8 codes, slot-major table), and the
-//     chain is then 32 v_readlane steps through those candidate vectors, the
-//     code being the lane of the next context.  v_writelane hands each
-//     position its (context, symbol) pair and the state update is
+//     chain through the round.  Each present context (K <= 64) owns a lane.
+//     Each lane copies the code row of its own state's slot (u8 codes of the
+//     symbol every context decodes there) into an LDS matrix, one row per
+//     position; each lane then follows four positions from its own context
+//     with three dependent LDS reads, so the chain across the 32 positions
+//     is 8 v_readlane steps (v_readlane issues about once per 25 clocks on
+//     gfx950, so the round avoids it everywhere else).  The state update is
 //     lane-parallel from a (context x symbol) table in LDS.  More than 64
-//     contexts spread each candidate vector over up to four VGPRs.
+//     contexts take a column gather and a 32-step chain with four contexts
+//     packed per lane.
 #include "rans_device.h"
 #include "rans_serve.h"
 
@@ -42,6 +44,9 @@ constexpr u32 OUT_ROUNDS = 32;                       // rounds per output flush
 constexpr u32 OUT_BYTES = OUT_ROUNDS * NSTATES;      // 1 KiB
 constexpr u32 UMAX = 64;                             // order-1 fast path lane space
 constexpr u32 FAST_ROUNDS = 4;                       // rounds per branch-free block
+// Order-1 code matrix rows: 64 codes plus 16 bytes of padding, so the
+// 16-byte row writes of consecutive lanes start on different LDS banks.
+constexpr u32 MROW = 80;
 
 struct DecShared {
     u32 tab[TOTFREQ];                                 // o0 slot table; o1: fc2[64][64]
@@ -52,6 +57,7 @@ struct DecShared {
     };
     u16 dense[256];                                   // order-1: context byte -> dense id
     u8 ulane[256];                                    // order-1: symbol -> lane in U
+    u8 mat[64 * MROW];                                // order-1: the round's code rows, one per position
 };
 
 // Words region reader (see the file comment).  Chunk c covers words
@@ -382,36 +388,49 @@ __device__ __forceinline__ u8 *table_base(u8 *ws)
 }
 
 // Fast path: K <= 64 contexts and |contexts u symbols| <= 64.
-// Workspace: t8[slot * K + d] (u8) = lane(sym), followed by 64 zero bytes, so
-// every lane of a gather (also lanes >= K, which read the next row) gets a
-// valid lane index.  LDS: fc2[d * 64 + lane(sym)] = (f-1) << 20 | cum << 8 | sym.
+// Workspace: t8[slot * S + d] (u8) = lane(sym), S = K rounded up to 16, the
+// bytes past K zero (context 0, a valid lane).  LDS: fc2[d * 64 + lane(sym)]
+// = (f-1) << 20 | cum << 8 | sym, and the round's matrix mat[j][c] (rows of
+// MROW bytes) = the code context c decodes to at position j.
 //
-// Per round lane c gathers, for each of the 32 slots, the lane of the symbol
-// context c decodes there; the chain then walks 32 dependent v_readlane
-// steps from the round's first context.  (Composing
-// the maps pairwise with ds_bpermute cuts the serial depth but costs more
-// issue slots than it saves: measured slower on gfx950.)
+// Per round: each lane loads 64 bytes at its slot's row (issued at the end
+// of the round before) and stores them as its matrix row; lane c then reads
+// mat[4g][c], mat[4g+1][.], mat[4g+2][.], mat[4g+3][.] along its own chain
+// (8 groups at once), packs the four codes, and the chain takes one
+// v_readlane per group.  Measured against a column gather (one v_readlane
+// for each slot's row offset, then 32 dependent v_readlane steps): 555 vs
+// 811 ns per round on one 4 MiB block, and composing the candidate maps with
+// ds_bpermute instead of LDS rows was slower than either.  Rounds run in
+// branch-free blocks of four with one check per block.
 __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
                               u64 hdr, u8 *ws)
 {
     const u32 l = lane_id();
     u8 *t8 = table_base(ws);
     u32 *fc2 = sh.tab;
+    // Rows of the code table are 16-byte aligned: S >= K bytes per slot.  A
+    // round reads 64 bytes from each row start: past S they are the codes of
+    // the next rows (valid lanes), and the last row's over-read stays inside
+    // the workspace (the table ends at least 1 MiB before it).
+    const u32 S = (K + 15) & ~15u;
+    // The last 16 bytes of every row are zeroed first, so the bytes past K
+    // read as context 0 (a valid lane) rather than a stale value.
+    for (u32 k = l; k < TOTFREQ; k += 64) *(uint4 *)(t8 + k * S + (S - 16)) = make_uint4(0, 0, 0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
     const u8 *t = s + HDR0 + BITMAP;
     for (u32 d = 0; d < K; d++, t += 512) {
         (void)load_freq_table(sh, t, end); // validated already
         for_each_slot(sh, [&](u32 k, u32 sym, u32 f, u32 cum) {
             const u32 ul = sh.ulane[sym];
-            t8[k * K + d] = (u8)ul;
+            t8[k * S + d] = (u8)ul;
             if (k == cum) fc2[d * UMAX + ul] = ((f - 1) << 20) | (cum << 8) | sym;
         });
         __syncthreads();
     }
-    t8[TOTFREQ * K + l] = 0;
     __threadfence();
     // Raw buffer view of t8 (gfx9 resource word 3), bounded to the table.
     const __amdgpu_buffer_rsrc_t t8rs =
-        __builtin_amdgcn_make_buffer_rsrc(t8, (short)0, (int)(TOTFREQ * K + 64), 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc(t8, (short)0, (int)(TOTFREQ * S), 0x00020000);
 
     WordRing wr;
     wr.init(s + hdr, end);
@@ -426,48 +445,96 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     u32 row = 0;
     u8 *o = out;
     u64 r = 0;
+    u8 *const mrow = sh.mat + MROW * l;
+    // Matrix bytes past the S copied per row stay zero (context 0).
+#pragma unroll
+    for (u32 i = 0; i < 4; i++) ((uint4 *)mrow)[i] = make_uint4(0, 0, 0, 0);
+    // Each lane copies the code row of its own slot (lanes 32..63 hold
+    // x = 0 and fill rows nobody reads) into row `l` of the LDS matrix:
+    // mat[j][c] is the code context c decodes to at position j.  Wide
+    // per-lane loads and stores, no cross-lane broadcast of addresses.  A
+    // round's rows are loaded as soon as its states are known, at the end of
+    // the round before, so their latency overlaps that round's bookkeeping.
+    typedef u32 v4u __attribute__((ext_vector_type(4)));
+    v4u q0, q1, q2, q3;
+#define LOAD_ROWS(xs)                                                                                  \
+    do {                                                                                               \
+        const v4u *src_ = (const v4u *)(t8 + (u64)((xs) & (TOTFREQ - 1)) * S);                        \
+        q0 = src_[0];                                                                                  \
+        q1 = src_[1];                                                                                  \
+        q2 = src_[2];                                                                                  \
+        q3 = src_[3];                                                                                  \
+    } while (0)
+    LOAD_ROWS(x);
     while (r < rounds) {
-        const u32 slot = x & (TOTFREQ - 1);
-        const u32 rowoff = slot * K;
-        u32 cand[NSTATES];
-        // Each slot's row offset is the scalar offset of a raw buffer load
-        // (no LDS round trip); lane l reads column l.
-#pragma unroll
-        for (u32 j = 0; j < NSTATES; j++)
-            cand[j] = __builtin_amdgcn_raw_buffer_load_b8(t8rs, l, __builtin_amdgcn_readlane(rowoff, j), 0);
-        if (r < full && prev < K && pos + NSTATES <= avail) {
-            // Fast round, no branch inside: a full round, a valid first
-            // context, no truncation possible and the ring holds every word
-            // the round can reach.  An absent context or a state needing a
-            // second word sends the round back to the exact path below.
-            const u32 x0 = x;
+        if (r + FAST_ROUNDS <= full && (row % FAST_ROUNDS) == 0) {
+            // FAST_ROUNDS rounds with no branch inside.  A round that meets an
+            // absent context, a state needing a second word or the end of
+            // the word stream marks its lanes bad; the whole block is then
+            // undone and redone by the exact path one round at a time.  The
+            // ring holds every word the block can reach (at most 128 words,
+            // the ring keeps 512 ahead of the chunk start); words read past
+            // the stream end are never kept (the block is undone).
+            const u32 x0 = x, prev0 = prev;
             const u64 pos0 = pos;
-            u32 c = prev;
-            u32 sv = 0;
+            u32 bad = 0;
 #pragma unroll
-            for (u32 j = 0; j < NSTATES; j++) {
-                c = __builtin_amdgcn_readlane(cand[j], c);
-                sv = writelane(c, j, sv);
+            for (u32 t = 0; t < FAST_ROUNDS; t++) {
+                const u32 slot = x & (TOTFREQ - 1);
+                ((v4u *)mrow)[0] = q0;
+                ((v4u *)mrow)[1] = q1;
+                ((v4u *)mrow)[2] = q2;
+                ((v4u *)mrow)[3] = q3;
+                __builtin_amdgcn_wave_barrier();
+                asm volatile("" ::: "memory");
+                // Lane c walks four positions from context c at position 4g
+                // (three dependent LDS reads per group, all groups at once),
+                // so the chain takes one v_readlane per group: dg holds the
+                // four codes with the last one in byte 0, where the next
+                // step's lane select reads it.
+                u32 dg[NSTATES / 4], e1[NSTATES / 4], e2[NSTATES / 4], e0[NSTATES / 4];
+#pragma unroll
+                for (u32 g = 0; g < NSTATES / 4; g++) e0[g] = sh.mat[MROW * (4 * g) + l];
+#pragma unroll
+                for (u32 g = 0; g < NSTATES / 4; g++) e1[g] = sh.mat[MROW * (4 * g + 1) + e0[g]];
+#pragma unroll
+                for (u32 g = 0; g < NSTATES / 4; g++) e2[g] = sh.mat[MROW * (4 * g + 2) + e1[g]];
+#pragma unroll
+                for (u32 g = 0; g < NSTATES / 4; g++) {
+                    const u32 e3 = sh.mat[MROW * (4 * g + 3) + e2[g]];
+                    dg[g] = e3 | (e0[g] << 8) | (e1[g] << 16) | (e2[g] << 24);
+                }
+                u32 c = prev, dv = 0, cv = 0;
+#pragma unroll
+                for (u32 g = 0; g < NSTATES / 4; g++) {
+                    cv = writelane(c, 4 * g, cv);
+                    c = __builtin_amdgcn_readlane(dg[g], c);
+                    dv = writelane(c, 4 * g, dv);
+                }
+                prev = c & 0xffu;
+                // Every lane of a group of four takes the group's dword; lane
+                // 4g + k reads its symbol's code and its context's code there.
+                const u32 dq = (u32)__builtin_amdgcn_update_dpp(0, (int)dv, 0x00, 0xf, 0xf, false);
+                const u32 cq = (u32)__builtin_amdgcn_update_dpp(0, (int)cv, 0x00, 0xf, 0xf, false);
+                const u32 k4 = l & 3;
+                const u32 sc = (dq >> (8 * ((k4 + 1) & 3))) & 0xffu;
+                const u32 cc = k4 == 0 ? (cq & 0xffu) : (dq >> (8 * k4)) & 0xffu;
+                const u32 ent = fc2[((cc & 63) << 6) | (sc & 63)];
+                const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
+                const bool need = lo && v < RANS_L;
+                const u64 m = ballot(need);
+                const u32 w = ((const u16 *)sh.ring)[(u32)(pos % RING_WORDS) + __builtin_amdgcn_mbcnt_lo((u32)m, 0u)];
+                if (lo) x = need ? ((v << 16) | w) : v;
+                pos += (u32)__popcll(m);
+                if (lo) stage_byte(sh, row + t, ent);
+                bad |= (u32)(lo && (x < RANS_L || cc >= K));
+                LOAD_ROWS(x);
             }
-            // Position j's context is position j-1's symbol: a one-lane shift
-            // within each 16-lane row (DPP row_shr:1) plus the two row heads.
-            u32 ctxv = (u32)__builtin_amdgcn_update_dpp((int)sv, (int)sv, 0x111, 0xf, 0xf, false);
-            ctxv = writelane(prev, 0, ctxv);
-            ctxv = writelane(__builtin_amdgcn_readlane(sv, 15), 16, ctxv);
-            const u32 cv = ctxv | (sv << 8);
-            const u32 ent = fc2[ctxv * UMAX + sv];
-            const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
-            const bool need = lo && v < RANS_L;
-            const u64 m = ballot(need);
-            const u32 w = ((const u16 *)sh.ring)[(u32)(pos % RING_WORDS) + __builtin_amdgcn_mbcnt_lo((u32)m, 0u)];
-            if (lo) x = need ? ((v << 16) | w) : v;
-            pos += (u32)__popcll(m);
-            if (lo) stage_byte(sh, row, ent);
-            if (ballot(lo && (x < RANS_L || (cv & 0xff) >= K)) == 0) {
-                prev = c;
-                r++;
+            if (ballot(bad != 0) == 0 && pos <= avail) {
+                r += FAST_ROUNDS;
+                row += FAST_ROUNDS;
                 wr.advance(sh.ring, pos);
-                if (++row == OUT_ROUNDS) {
+                if (row == OUT_ROUNDS) {
                     flush(sh, o, OUT_BYTES);
                     o += OUT_BYTES;
                     row = 0;
@@ -475,8 +542,17 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
                 continue;
             }
             x = x0;
+            prev = prev0;
             pos = pos0;
+            LOAD_ROWS(x);
         }
+        const u32 slot = x & (TOTFREQ - 1);
+        ((v4u *)mrow)[0] = q0;
+        ((v4u *)mrow)[1] = q1;
+        ((v4u *)mrow)[2] = q2;
+        ((v4u *)mrow)[3] = q3;
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
         // Exact path: absent contexts, truncation, multi-word states, tail.
         if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT;
         const u32 cntr = r < full ? NSTATES : (u32)(n - r * NSTATES);
@@ -484,7 +560,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
         u32 cv = 0;
 #pragma unroll
         for (u32 j = 0; j < NSTATES; j++) {
-            const u32 nxt = __builtin_amdgcn_readlane(cand[j], c);
+            const u32 nxt = sh.mat[MROW * j + (c & 63)];
             cv = writelane(c | (nxt << 8), j, cv);
             c = nxt;
         }
@@ -499,6 +575,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
         if (err) return err;
         if (act) stage_byte(sh, row, ent);
         if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        LOAD_ROWS(x);
         r++;
         if (++row == OUT_ROUNDS) {
             // The last round may be partial: never write past out + n.
@@ -510,6 +587,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     }
     if (row) flush(sh, o, (u32)(n - (u64)(o - out)));
     return 0;
+#undef LOAD_ROWS
 }
 
 // Wide path: more than 64 contexts, or 64 contexts and one more symbol

@@ -21,8 +21,9 @@
 //   * order-1: symbol i's context is symbol i-1 (scalar.rs:203), a serial
 //     chain through the round.  Each present context (K <= 64) owns a lane.
 //     Each lane copies the code row of its own state's slot (u8 codes of the
-//     symbol every context decodes there) into an LDS matrix, one row per
-//     position; each lane then follows four positions from its own context
+//     symbol every context decodes there; the rows are stored once per
+//     distinct row, found through a slot -> row index in LDS) into an LDS
+//     matrix, one row per position; each lane then follows four positions from its own context
 //     with three dependent LDS reads, so the chain across the 32 positions
 //     is 8 v_readlane steps (v_readlane issues about once per 25 clocks on
 //     gfx950, so the round avoids it everywhere else).  The state update is
@@ -58,6 +59,8 @@ struct DecShared {
     u16 dense[256];                                   // order-1: context byte -> dense id
     u8 ulane[256];                                    // order-1: symbol -> lane in U
     u8 mat[64 * MROW];                                // order-1: the round's code rows, one per position
+                                                      // (set-up: slot boundary bitmap, 128 words)
+    alignas(16) u16 rowidx[TOTFREQ];                  // order-1: slot -> distinct code row
 };
 
 // Words region reader (see the file comment).  Chunk c covers words
@@ -346,10 +349,16 @@ __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
 // Block setup shared by both order-1 paths: presence bitmap, then every
 // table read and validated in ascending context order (scalar.rs:181-194).
 // Leaves K, sh.dense, and the symbol presence of lane l's symbols in symp.
+// Also marks in the first 128 words of sh.mat every slot > 0 at which some
+// context's symbol starts (its cum): the code row of a slot, the symbol every
+// context decodes there, changes only at those slots.
 __device__ int o1_validate(DecShared &sh, const u8 *s, const u8 *end, u32 &K, u32 &symp)
 {
     const u32 l = lane_id();
     const u64 len = (u64)(end - s);
+    u32 *bnd = (u32 *)sh.mat;
+    bnd[2 * l] = 0;
+    bnd[2 * l + 1] = 0;
     if (len < HDR0 + BITMAP) return FQXV_RANS_ERR_TRUNCATED;
     const u32 bm = s[HDR0 + l / 2];
     const u32 bits = (bm >> (4 * (l & 1))) & 0xf;
@@ -369,8 +378,13 @@ __device__ int o1_validate(DecShared &sh, const u8 *s, const u8 *end, u32 &K, u3
         if ((u64)(t + 512 - s) > len) return FQXV_RANS_ERR_TRUNCATED;
         int err = load_freq_table(sh, t, end);
         if (err) return err;
-        for (int i = 0; i < 4; i++) symp |= (u32)((sh.fc[4 * l + i] >> 16) != 0) << i;
+        for (int i = 0; i < 4; i++) {
+            const u32 fcv = sh.fc[4 * l + i], cum = fcv & 0xffff;
+            symp |= (u32)((fcv >> 16) != 0) << i;
+            if ((fcv >> 16) != 0 && cum != 0) atomicOr(&bnd[cum >> 5], 1u << (cum & 31));
+        }
     }
+    __syncthreads();
     return 0;
 }
 
@@ -388,49 +402,106 @@ __device__ __forceinline__ u8 *table_base(u8 *ws)
 }
 
 // Fast path: K <= 64 contexts and |contexts u symbols| <= 64.
-// Workspace: t8[slot * S + d] (u8) = lane(sym), S = K rounded up to 16, the
-// bytes past K zero (context 0, a valid lane).  LDS: fc2[d * 64 + lane(sym)]
-// = (f-1) << 20 | cum << 8 | sym, and the round's matrix mat[j][c] (rows of
-// MROW bytes) = the code context c decodes to at position j.
+// Workspace: t8[row * S + d] (u8) = lane(sym) for each distinct code row, S =
+// K rounded up to 16, the bytes past K zero (context 0, a valid lane).  LDS:
+// rowidx[slot] = the slot's row, fc2[d * 64 + lane(sym)] = (f-1) << 20 |
+// cum << 8 | sym, and the round's matrix mat[j][c] (rows of MROW bytes) = the
+// code context c decodes to at position j.
 //
-// Per round: each lane loads 64 bytes at its slot's row (issued at the end
-// of the round before) and stores them as its matrix row; lane c then reads
-// mat[4g][c], mat[4g+1][.], mat[4g+2][.], mat[4g+3][.] along its own chain
-// (8 groups at once), packs the four codes, and the chain takes one
-// v_readlane per group.  Measured against a column gather (one v_readlane
-// for each slot's row offset, then 32 dependent v_readlane steps): 555 vs
-// 811 ns per round on one 4 MiB block, and composing the candidate maps with
-// ds_bpermute instead of LDS rows was slower than either.  Rounds run in
-// branch-free blocks of four with one check per block.
+// Per round: each lane looks up its slot's row and loads 64 bytes there
+// (issued at the end of the round before) and stores them as its matrix
+// row; lane c then reads mat[4g][c], mat[4g+1][.], mat[4g+2][.],
+// mat[4g+3][.] along its own chain (8 groups at once), packs the four codes,
+// and the chain takes one v_readlane per group.  Measured against a column
+// gather (one v_readlane for each slot's row offset, then 32 dependent
+// v_readlane steps): 555 vs 811 ns per round on one 4 MiB block, and
+// composing the candidate maps with ds_bpermute instead of LDS rows was
+// slower than either.  Rounds run in branch-free blocks of four with one
+// check per block.  A table of one row per slot (4096 * S bytes, 196 KiB at
+// K = 41) was read the same way; with four blocks per CU decoding at once
+// those tables overflowed L2 (hit rate 40% against 97% at one per CU) and a
+// round slowed from 555 to 810 ns.  The distinct rows number 21 (DNA), about
+// 100 (names) and about 300 (K = 41 qualities) on the test corpora.
 __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
                               u64 hdr, u8 *ws)
 {
     const u32 l = lane_id();
     u8 *t8 = table_base(ws);
     u32 *fc2 = sh.tab;
-    // Rows of the code table are 16-byte aligned: S >= K bytes per slot.  A
-    // round reads 64 bytes from each row start: past S they are the codes of
-    // the next rows (valid lanes), and the last row's over-read stays inside
-    // the workspace (the table ends at least 1 MiB before it).
+    // Distinct code rows.  The row of slot k (the code every context decodes
+    // at k) changes only at the boundary slots o1_validate marked, so the
+    // rows are numbered in slot order, rowidx[k] = number of boundaries in
+    // [1, k], and t8 holds one row per number: at most 1 + sum over contexts
+    // of their symbol counts (tens to a few hundred), against 4096 per slot.
+    // The small table stays in L1/L2 while many blocks decode at once.
+    {
+        const u32 *bm = (const u32 *)sh.mat;
+        const u32 w0 = bm[2 * l], w1 = bm[2 * l + 1];
+        const u32 cnt = __popc(w0) + __popc(w1);
+        u32 base = wave_scan_incl(cnt) - cnt;
+#pragma unroll
+        for (u32 h = 0; h < 2; h++) {
+            const u32 w = h ? w1 : w0;
+#pragma unroll
+            for (u32 q = 0; q < 4; q++) {
+                u32 v[4];
+#pragma unroll
+                for (u32 i = 0; i < 4; i++) {
+                    const u32 b = 8 * q + 2 * i; // slots 64l + 32h + b, b + 1
+                    const u32 lo16 = base + __popc(w & ((2u << b) - 1));
+                    const u32 hi16 = base + __popc(w & ((2u << (b + 1)) - 1));
+                    v[i] = lo16 | (hi16 << 16);
+                }
+                *(uint4 *)&sh.rowidx[64 * l + 32 * h + 8 * q] = make_uint4(v[0], v[1], v[2], v[3]);
+            }
+            base += __popc(w);
+        }
+    }
+    const u32 nrows = uni(sh.rowidx[TOTFREQ - 1]) + 1;
+    // Rows are 16-byte aligned: S >= K bytes each.  A round reads 64 bytes
+    // from each row start: past S they are the codes of the next rows, and
+    // the last row's over-read stays inside the workspace (the table ends at
+    // least 1 MiB before it).  The last 16 bytes of every row are zeroed
+    // first, so the bytes past K read as context 0 rather than a stale value.
     const u32 S = (K + 15) & ~15u;
-    // The last 16 bytes of every row are zeroed first, so the bytes past K
-    // read as context 0 (a valid lane) rather than a stale value.
-    for (u32 k = l; k < TOTFREQ; k += 64) *(uint4 *)(t8 + k * S + (S - 16)) = make_uint4(0, 0, 0, 0);
+    for (u32 i = l; i < nrows; i += 64) *(uint4 *)(t8 + i * S + (S - 16)) = make_uint4(0, 0, 0, 0);
     __builtin_amdgcn_s_waitcnt(0);
-    const u8 *t = s + HDR0 + BITMAP;
-    for (u32 d = 0; d < K; d++, t += 512) {
-        (void)load_freq_table(sh, t, end); // validated already
-        for_each_slot(sh, [&](u32 k, u32 sym, u32 f, u32 cum) {
-            const u32 ul = sh.ulane[sym];
-            t8[k * S + d] = (u8)ul;
-            if (k == cum) fc2[d * UMAX + ul] = ((f - 1) << 20) | (cum << 8) | sym;
-        });
-        __syncthreads();
+    {
+        const u32 *bm = (const u32 *)sh.mat;
+        // Row starts in this lane's slots [64l, 64l + 64): the boundaries,
+        // and slot 0 (row 0) on lane 0.
+        const u64 starts = ((u64)bm[2 * l + 1] << 32 | bm[2 * l]) | (l == 0 ? 1ull : 0ull);
+        const u32 row0 = l == 0 ? 0u : (u32)sh.rowidx[64 * l] - (u32)(bm[2 * l] & 1u) + 1u;
+        const u8 *t = s + HDR0 + BITMAP;
+        for (u32 d = 0; d < K; d++, t += 512) {
+            (void)load_freq_table(sh, t, end); // validated already
+#pragma unroll
+            for (u32 i = 0; i < 4; i++) {
+                const u32 sym = 4 * l + i, fcv = sh.fc[sym], f = fcv >> 16;
+                if (f) fc2[d * UMAX + sh.ulane[sym]] = ((f - 1) << 20) | ((fcv & 0xffff) << 8) | sym;
+            }
+            if (starts) {
+                // Owner of slot 64l: the largest symbol with cum <= 64l.
+                const u32 k0 = 64 * l;
+                u32 lo = 0, hi = 255;
+                while (lo < hi) {
+                    const u32 mid = (lo + hi + 1) / 2;
+                    if ((sh.fc[mid] & 0xffff) <= k0)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                u32 sy = lo, fcs = sh.fc[sy], ri = row0;
+                for (u64 rs = starts; rs; rs &= rs - 1, ri++) {
+                    const u32 k = k0 + (u32)__builtin_ctzll(rs);
+                    while ((fcs & 0xffff) + (fcs >> 16) <= k) fcs = sh.fc[++sy];
+                    t8[ri * S + d] = sh.ulane[sy];
+                }
+            }
+            __syncthreads();
+        }
     }
     __threadfence();
-    // Raw buffer view of t8 (gfx9 resource word 3), bounded to the table.
-    const __amdgpu_buffer_rsrc_t t8rs =
-        __builtin_amdgcn_make_buffer_rsrc(t8, (short)0, (int)(TOTFREQ * S), 0x00020000);
 
     WordRing wr;
     wr.init(s + hdr, end);
@@ -459,7 +530,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     v4u q0, q1, q2, q3;
 #define LOAD_ROWS(xs)                                                                                  \
     do {                                                                                               \
-        const v4u *src_ = (const v4u *)(t8 + (u64)((xs) & (TOTFREQ - 1)) * S);                        \
+        const v4u *src_ = (const v4u *)(t8 + (u32)sh.rowidx[(xs) & (TOTFREQ - 1)] * S);               \
         q0 = src_[0];                                                                                  \
         q1 = src_[1];                                                                                  \
         q2 = src_[2];                                                                                  \
@@ -492,7 +563,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
                 // so the chain takes one v_readlane per group: dg holds the
                 // four codes with the last one in byte 0, where the next
                 // step's lane select reads it.
-                u32 dg[NSTATES / 4], e1[NSTATES / 4], e2[NSTATES / 4], e0[NSTATES / 4];
+                u32 dg[NSTATES / 4], e1[NSTATES / 4], e2[NSTATES / 4], e0[NSTATES / 4], e3[NSTATES / 4];
 #pragma unroll
                 for (u32 g = 0; g < NSTATES / 4; g++) e0[g] = sh.mat[MROW * (4 * g) + l];
 #pragma unroll
@@ -500,10 +571,13 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
 #pragma unroll
                 for (u32 g = 0; g < NSTATES / 4; g++) e2[g] = sh.mat[MROW * (4 * g + 2) + e1[g]];
 #pragma unroll
-                for (u32 g = 0; g < NSTATES / 4; g++) {
-                    const u32 e3 = sh.mat[MROW * (4 * g + 3) + e2[g]];
-                    dg[g] = e3 | (e0[g] << 8) | (e1[g] << 16) | (e2[g] << 24);
-                }
+                for (u32 g = 0; g < NSTATES / 4; g++) e3[g] = sh.mat[MROW * (4 * g + 3) + e2[g]];
+                // All eight last reads in flight before the first is used (the
+                // scheduler otherwise waits on each in turn in some kernels).
+                asm volatile("" : "+v"(e3[0]), "+v"(e3[1]), "+v"(e3[2]), "+v"(e3[3]), "+v"(e3[4]), "+v"(e3[5]),
+                             "+v"(e3[6]), "+v"(e3[7]));
+#pragma unroll
+                for (u32 g = 0; g < NSTATES / 4; g++) dg[g] = e3[g] | (e0[g] << 8) | (e1[g] << 16) | (e2[g] << 24);
                 u32 c = prev, dv = 0, cv = 0;
 #pragma unroll
                 for (u32 g = 0; g < NSTATES / 4; g++) {
@@ -516,9 +590,12 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
                 // 4g + k reads its symbol's code and its context's code there.
                 const u32 dq = (u32)__builtin_amdgcn_update_dpp(0, (int)dv, 0x00, 0xf, 0xf, false);
                 const u32 cq = (u32)__builtin_amdgcn_update_dpp(0, (int)cv, 0x00, 0xf, 0xf, false);
+                // Lane 4g takes its context from cv (the group's entry
+                // context), lanes 4g+1..3 from dv: one byte select, no lane
+                // mask (a loop-invariant mask costs SGPRs the kernel spills).
                 const u32 k4 = l & 3;
                 const u32 sc = (dq >> (8 * ((k4 + 1) & 3))) & 0xffu;
-                const u32 cc = k4 == 0 ? (cq & 0xffu) : (dq >> (8 * k4)) & 0xffu;
+                const u32 cc = (((dq & 0xffffff00u) | (cq & 0xffu)) >> (8 * k4)) & 0xffu;
                 const u32 ent = fc2[((cc & 63) << 6) | (sc & 63)];
                 const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
                 const bool need = lo && v < RANS_L;

@@ -23,13 +23,13 @@
 //     Each lane copies the code row of its own state's slot (u8 codes of the
 //     symbol every context decodes there; the rows are stored once per
 //     distinct row, found through a slot -> row index in LDS) into an LDS
-//     matrix, one row per position; each lane then follows four positions from its own context
-//     with three dependent LDS reads, so the chain across the 32 positions
-//     is 8 v_readlane steps (v_readlane issues about once per 25 clocks on
-//     gfx950, so the round avoids it everywhere else).  The state update is
-//     lane-parallel from a (context x symbol) table in LDS.  More than 64
-//     contexts take a column gather and a 32-step chain with four contexts
-//     packed per lane.
+//     matrix, one row per position; each lane then follows four positions
+//     from its own context with three dependent LDS reads, so the chain
+//     across the 32 positions is 8 v_readlane steps (v_readlane issues
+//     about once per 25 clocks on gfx950, so the round avoids it everywhere
+//     else).  The state update is lane-parallel from a (context x symbol)
+//     table in LDS.  More than 64 contexts take a column gather and a
+//     32-step chain with four contexts packed per lane.
 #include "rans_device.h"
 #include "rans_serve.h"
 

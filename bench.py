@@ -637,22 +637,21 @@ def bench_names(coder, blocks=32, per_block=40000, distinct=8):
 
 
 # ------------------------------------------------------------------------ C5
-def bench_serve(args, coder, dev, state):
+def serve_peak(dev, enc, comp_len, corpus, arena, waves, peak_blocks):
+    """Batched decode rate of this corpus on `waves` decode waves: one graph
+    launch over copies of every block, on a context of its own."""
     import torch
-    from fqxv_amd.serve import Server, poisson_requests, run_load, summarize
-    corpus, arena, enc, comp_len = state
+    import fqxv_amd
     nb = len(corpus.sizes)
     blk = int(corpus.sizes[0])
-    # batched peak on this corpus: one launch over R copies of every block
-    reps = max(1, args.peak_blocks // nb)
+    reps = max(1, peak_blocks // nb)
     k = nb * reps
-    src_off = np.tile(enc.dst_off, reps)
-    src_len = np.tile(comp_len, reps)
-    sizes = np.tile(corpus.sizes, reps)
+    pc = fqxv_amd.Coder(dev.index or 0)
+    pc.set_grid(waves)
     d_off = np.arange(k, dtype=np.uint64) * np.uint64(blk)
     dst = torch.empty(k * blk + 64, dtype=torch.uint8, device=dev)
-    jobs = coder.jobs_array(src_off, src_len, d_off, sizes)
-    g = coder.graph("decode", enc.dst, dst, jobs, None)
+    jobs = pc.jobs_array(np.tile(enc.dst_off, reps), np.tile(comp_len, reps), d_off, np.tile(corpus.sizes, reps))
+    g = pc.graph("decode", enc.dst, dst, jobs, None)
     g.launch()
     stream = torch.cuda.current_stream()
     ev = events(2)
@@ -663,20 +662,39 @@ def bench_serve(args, coder, dev, state):
     torch.cuda.synchronize()
     assert int((g.status != 0).sum()) == 0
     assert torch.equal(dst[(k - 1) * blk:k * blk], arena[(nb - 1) * blk:nb * blk])
-    t_peak = min(a.elapsed_time(b) for a, b in ev) / 1e3
-    peak = k * blk / t_peak
+    t = min(a.elapsed_time(b) for a, b in ev) / 1e3
     g.close()
-    del dst
+    del dst, pc
     torch.cuda.empty_cache()
-    log(f"[serve] batched peak {peak / 1e9:.2f} GB/s ({k} blocks in {t_peak * 1e3:.1f} ms)")
+    return k * blk / t, k, t
 
+
+def bench_serve(args, coder, dev, state):
+    import torch
+    from fqxv_amd.serve import Server, poisson_requests, run_load, summarize
+    corpus, arena, enc, comp_len = state
+    nb = len(corpus.sizes)
+    blk = int(corpus.sizes[0])
     budget = args.serve_budget_gib << 30
     mean_req = 32.5 * blk
     bad = []
-    modes = {}
+    modes, peaks, single = {}, {}, {}
     for mode in args.serve_modes:
         srv = Server(enc.dst, enc.dst_off, comp_len, corpus.sizes, budget=budget, slots=args.serve_slots,
                      max_batch=4096, mode=mode, waves=args.serve_waves)
+        waves = srv.waves
+        # The offered-load axis is relative to a batched decode on the same
+        # number of decode waves as this engine runs.
+        peak, k, t_peak = serve_peak(dev, enc, comp_len, corpus, arena, waves, args.peak_blocks)
+        log(f"[serve:{mode}] {waves} decode waves; batched peak on them {peak / 1e9:.2f} GB/s "
+            f"({k} blocks in {t_peak * 1e3:.1f} ms)")
+        # One-block requests on an idle engine: the latency floor of a request.
+        one = []
+        for b in range(min(8, nb)):
+            recs = run_load(srv, [0.0], [np.array([b], np.uint32)])
+            one.append(recs[0].latency * 1e3)
+        single[mode] = round(float(np.median(one)), 3)
+        peaks[mode] = {"waves": waves, "gbs": round(peak / 1e9, 3), "blocks": k, "ms": round(t_peak * 1e3, 2)}
         points = []
         for load in args.serve_loads:
             rate = load * peak / mean_req
@@ -692,16 +710,22 @@ def bench_serve(args, coder, dev, state):
             recs = run_load(srv, arrivals, reqs, on_done=verify)
             s = summarize(recs, lambda i, reqs=reqs: len(reqs[i]) * blk)
             s.update({"offered_load": load, "offered_gbs": round(rate * mean_req / 1e9, 3),
+                      "p99_over_single_block": round(s["p99_ms"] / single[mode], 3),
+                      "gbs_over_same_grid_peak": round(s["gbs"] * 1e9 / peak, 3),
                       "max_in_flight": srv.max_in_flight, "max_queued": srv.max_queued})
             points.append(s)
-            log(f"[serve:{mode}] load {load:.0%}: p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms, {s['gbs']} GB/s, "
-                f"in flight <= {srv.max_in_flight}, queued <= {srv.max_queued}")
+            log(f"[serve:{mode}] load {load:.0%}: p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms "
+                f"({s['p99_over_single_block']}x one block), {s['gbs']} GB/s "
+                f"({s['gbs_over_same_grid_peak']:.0%} of peak), in flight <= {srv.max_in_flight}, "
+                f"queued <= {srv.max_queued}")
         srv.close()
         modes[mode] = points
     assert not bad, f"serving outputs differ: {bad[:5]}"
     assert all(p["errors"] == 0 for pts in modes.values() for p in pts)
     first = args.serve_modes[0]
-    return {"corpus": "markov_64x4mib, resident compressed in HBM", "batched_peak_gbs": round(peak / 1e9, 3),
+    return {"corpus": "markov_64x4mib, resident compressed in HBM", "batched_peak_gbs": peaks[first]["gbs"],
+            "peak": "batched decode (one graph launch over copies of every block) on the engine's own decode waves",
+            "peaks": peaks, "single_block_ms": single,
             "requests_per_point": args.serve_requests, "request_blocks": "uniform 1-64 distinct ids, seed 5",
             "budget_gib": args.serve_budget_gib, "mode": first, "points": modes[first],
             "modes": {"persistent": "one resident grid fed block by block from a host-appended ring",
@@ -771,7 +795,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--peak-blocks", type=int, default=2048)
-    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.1, 0.25, 0.5, 0.75, 1.0, 1.2])
     ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--serve-slots", type=int, default=4)

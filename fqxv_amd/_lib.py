@@ -39,7 +39,7 @@ EXPORTS = [
     "fqxv_rans_graph_create", "fqxv_rans_graph_launch", "fqxv_rans_graph_destroy",
     "fqxv_rans_ctx_set_grid", "fqxv_rans_server_create", "fqxv_rans_server_destroy",
     "fqxv_rans_server_now", "fqxv_rans_server_submit",
-    "fqxv_rans_server_poll", "fqxv_rans_server_take", "fqxv_rans_server_release",
+    "fqxv_rans_server_poll", "fqxv_rans_server_take", "fqxv_rans_server_release", "fqxv_rans_server_waves",
     "fqxv_rans_encode_best_batch", "fqxv_rans_encode_best_bound",
     "fqxv_rans_perm_split", "fqxv_rans_perm_join", "fqxv_rans_crc32c_batch",
     "fqxv_names_decode_batch", "fqxv_names_status", "fqxv_names_records",
@@ -131,6 +131,7 @@ def load():
         L.fqxv_rans_server_poll.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]
         L.fqxv_rans_server_take.argtypes = [vp, ctypes.POINTER(Done), u32, ctypes.POINTER(u32)]
         L.fqxv_rans_server_release.argtypes = [vp, ctypes.c_uint64]
+        L.fqxv_rans_server_waves.argtypes = [vp, ctypes.POINTER(u32)]
         L.fqxv_names_decode_batch.argtypes = [vp, vp, vp, vp, u32, ctypes.POINTER(vp)]
         L.fqxv_names_status.argtypes = [vp, u32]
         L.fqxv_names_status.restype = ctypes.c_int32

@@ -26,7 +26,8 @@ struct fqxv_serve_dev {   // device memory; the claim counter and the read-mostl
                           // published count sit on separate 128-byte lines
     uint64_t head;        // jobs claimed so far (compare-and-swap)
     uint32_t alive;       // decode waves still running
-    uint32_t pad0[29];
+    uint32_t disp_cu;     // the dispatcher's compute unit (set once, 0 before)
+    uint32_t pad0[28];
     uint64_t dtail;       // device copy of the published count
     uint32_t dstop;       // device copy of stop
     uint32_t pad1[29];

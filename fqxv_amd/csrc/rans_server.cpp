@@ -376,6 +376,7 @@ int admit_p(fqxv_rans_server *s)
     s->h_dev->dtail = s->pub;
     s->h_dev->dstop = 0;
     s->h_dev->alive = s->waves;
+    s->h_dev->disp_cu = 0;
     if (hipMemcpyAsync(s->d_dev, s->h_dev, sizeof(fqxv_serve_dev), hipMemcpyHostToDevice, s->pstream) != hipSuccess ||
         fqxv_launch_serve(s->d_archive, s->d_pool, s->dp_jobs, s->dp_res, s->ring, s->dp_ctl, s->d_dev, s->d_ws,
                           s->idle_ticks, s->waves, s->pstream) != hipSuccess ||
@@ -597,6 +598,22 @@ int fqxv_rans_server_release(fqxv_rans_server *s, uint64_t ticket)
     if (it == s->held.end()) return FQXV_RANS_ERR_INVALID_ARG;
     s->pool->release(it->second.first, it->second.second);
     s->held.erase(it);
+    return FQXV_RANS_OK;
+}
+
+int fqxv_rans_server_waves(fqxv_rans_server *s, uint32_t *waves)
+{
+    if (!s || !waves) return FQXV_RANS_ERR_INVALID_ARG;
+    if (s->persistent) {
+        *waves = s->waves;
+        return FQXV_RANS_OK;
+    }
+    DeviceGuard dg(s->device);
+    hipDeviceProp_t prop;
+    int occ = 0;
+    if (hipGetDeviceProperties(&prop, s->device) != hipSuccess || fqxv_decode_occupancy(&occ) != hipSuccess)
+        return FQXV_RANS_ERR_DEVICE;
+    *waves = (uint32_t)std::max(1, occ) * (uint32_t)prop.multiProcessorCount;
     return FQXV_RANS_OK;
 }
 

@@ -118,6 +118,13 @@ class Server:
     def release(self, ticket: int) -> None:
         _raise(self._L.fqxv_rans_server_release(self._h, int(ticket)))
 
+    @property
+    def waves(self) -> int:
+        """Decode waves the service runs (resident grid, or each batched launch)."""
+        w = ctypes.c_uint32()
+        _raise(self._L.fqxv_rans_server_waves(self._h, ctypes.byref(w)))
+        return int(w.value)
+
     def block_offsets(self, ids) -> np.ndarray:
         """Offsets of a request's blocks inside its output region."""
         r = (self.n[np.asarray(ids, dtype=np.int64)] + 255) // 256 * 256

@@ -255,6 +255,10 @@ int fqxv_rans_server_poll(fqxv_rans_server *srv, uint32_t *queued, uint32_t *in_
 int fqxv_rans_server_take(fqxv_rans_server *srv, fqxv_rans_done *out, uint32_t max, uint32_t *got);
 /* Returns a completed request's output region to the pool. */
 int fqxv_rans_server_release(fqxv_rans_server *srv, uint64_t ticket);
+/* Decode waves the service runs: the resident grid's (persistent form) or
+ * each batched launch's (occupancy maximum), so a batched peak can be
+ * measured on the same number of waves. */
+int fqxv_rans_server_waves(fqxv_rans_server *srv, uint32_t *waves);
 
 /* The reorder permutation's byte planes around the coder
  * (crates/fqxv/src/container/reorder.rs:597-607 encode side,

@@ -199,6 +199,17 @@ struct TabRef {
 };
 
 
+// All 16 values of a batch live before any is used: the table reads of a
+// batch then share one wait (the scheduler otherwise waits on each read in
+// turn in the larger kernels).
+static_assert(WALK_BATCH == 16, "pin16 covers one batch");
+__device__ __forceinline__ void pin16(u32 (&a)[16])
+{
+    asm volatile("" : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]),
+                 "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]),
+                 "+v"(a[14]), "+v"(a[15]));
+}
+
 template <u32 MODE>
 __device__ __forceinline__ u32 tab_index_t(const TabRef &t, u32 ctx, u32 sym)
 {
@@ -282,7 +293,8 @@ __device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg, g
 // read before their state updates, so the LDS latency is paid once per batch
 // on the state chain.
 template <u32 MODE>
-__device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u8 *trash_generic, u32 &x_out)
+__device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend,
+                                     u8 *trash_generic, u32 &x_out)
 {
     gu16 *const trash = (gu16 *)trash_generic;
     gu16 *const wtop = (gu16 *)wend;
@@ -302,11 +314,32 @@ __device__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *we
         const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
         u64 r = r_hi + 1; // rounds r-1 down to r_lo remain
         while (r >= r_lo + WALK_BATCH) {
-            u32 fcm[WALK_BATCH], mg[WALK_BATCH];
+            // Three phases, each's reads in flight together: the staged
+            // bytes, then the table entries they index, then the steps.
+            u32 sy[WALK_BATCH], cx[WALK_BATCH], fv[WALK_BATCH], mv[WALK_BATCH];
 #pragma unroll
-            for (u32 k = 0; k < WALK_BATCH; k++) round_entry<MODE>(sh, tr, sb, p0, r - 1 - k, n, fcm[k], mg[k]);
+            for (u32 k = 0; k < WALK_BATCH; k++) {
+                const u32 t = (u32)((r - 1 - k) * NSTATES + (lane_id() & (NSTATES - 1)) - p0);
+                sy[k] = sb[4 + t];
+                cx[k] = MODE == 0 ? 0u : sb[3 + t];
+            }
+            pin16(sy);
+            pin16(cx);
 #pragma unroll
-            for (u32 k = 0; k < WALK_BATCH; k++) round_step(x, wp, fcm[k], mg[k], trash);
+            for (u32 k = 0; k < WALK_BATCH; k++) {
+                const bool act = lane_id() < NSTATES && (r - 1 - k) * NSTATES + lane_id() < n;
+                u32 e = MODE == 0 ? sy[k] : (MODE == 1 ? cx[k] * tr.cols + sy[k] : cx[k] * 256 + sy[k]);
+                e = act ? e : 0u;
+                fv[k] = MODE == 2 ? tr.fc[e] : sh.fc[e];
+                mv[k] = MODE == 2 ? tr.mg[e] : sh.mg[e];
+            }
+            pin16(fv);
+            pin16(mv);
+#pragma unroll
+            for (u32 k = 0; k < WALK_BATCH; k++) {
+                const bool act = lane_id() < NSTATES && (r - 1 - k) * NSTATES + lane_id() < n;
+                round_step(x, wp, act ? fv[k] : (TOTFREQ << 12), act ? mv[k] : 1u, trash);
+            }
             r -= WALK_BATCH;
         }
         while (r > r_lo) {
@@ -336,6 +369,126 @@ __device__ void finish(u8 *dst, u64 hdr, u8 *wend, u64 W, u32 x)
     wave_copy_fwd(dst + hdr, wend - 2 * W, 2 * W);
 }
 
+// Both orders in one reverse walk (encode-best when neither order's size
+// bounds win outright): lanes 0..31 carry the order-1 states, lanes 32..63
+// the order-0 states of the same positions.  Dense order-1 only: the staged
+// bytes are dense ids, the order-1 entry is ctx * cols + sym and the order-0
+// entry base0 + sym, both in sh.fc / sh.mg.  Each half has its own word
+// pointer; a lane's rank counts the emitting lanes below it in its half.
+__device__ __forceinline__ void round_entry_both(const EncShared &sh, const u8 *sb, u64 p0, u64 r, u64 n,
+                                                 u32 cols, u32 base0, u32 &fcm, u32 &mg)
+{
+    const u32 l = lane_id();
+    const u64 i = r * NSTATES + (l & (NSTATES - 1));
+    const bool act = i < n;
+    const u32 t = (u32)(i - p0);
+    const u32 sym = sb[4 + t];
+    const u32 ctx = sb[3 + t];
+    u32 e = l < NSTATES ? ctx * cols + sym : base0 + sym;
+    e = act ? e : 0u;
+    const u32 fv = sh.fc[e], mv = sh.mg[e];
+    fcm = act ? fv : (TOTFREQ << 12);
+    mg = act ? mv : 1u;
+}
+
+__device__ __forceinline__ void round_step_both(u32 &x, gu16 *&wp1, gu16 *&wp0, u32 fcm, u32 mg, gu16 *trash)
+{
+    const u32 l = lane_id();
+    const u32 f = fcm >> 12, cum = fcm & 0xfff;
+    const bool need = (x >> 20) >= f;
+    const u32 nb = (f - 1u - (x >> 20)) >> 31;
+    const u32 v = x >> (nb << 4);
+    const u64 m = ballot(need);
+    const u32 k1 = (u32)__popc((u32)m), k0 = (u32)__popc((u32)(m >> 32));
+    wp1 -= k1;
+    wp0 -= k0;
+    // Lanes below 32: rank among the order-1 half; from 32: among the order-0
+    // half (the full count minus the order-1 half's words).
+    const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+    const u32 hi = l >> 5;
+    const u32 rank = below - (k1 & (0u - hi));
+    const u64 w1 = (u64)(uintptr_t)wp1, w0 = (u64)(uintptr_t)wp0;
+    const u64 aw = (w1 ^ ((w1 ^ w0) & (0ull - (u64)hi))) + 2ull * rank;
+    const u64 at = (u64)(uintptr_t)(trash + l);
+    *(gu16 *)(uintptr_t)(at ^ ((aw ^ at) & (0ull - (u64)nb))) = (u16)x;
+    u32 q = __umulhi(v, mg);
+    u32 rr = v - q * f;
+    if (rr >= f) {
+        q++;
+        rr -= f;
+    }
+    x = (q << SCALE_BITS) + rr + cum;
+}
+
+// Returns the order-1 and order-0 word counts; the words end at wend1 and
+// wend0 (2-byte aligned); x_out: order-1 states in lanes 0..31, order-0 in
+// lanes 32..63.
+__device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u32 cols, u32 base0, u8 *wend1, u8 *wend0,
+                          u8 *trash_generic, u32 &x_out, u64 &W1, u64 &W0)
+{
+    gu16 *const trash = (gu16 *)trash_generic;
+    gu16 *const top1 = (gu16 *)wend1, *const top0 = (gu16 *)wend0;
+    gu16 *wp1 = top1, *wp0 = top0;
+    u32 x = RANS_L;
+    const u64 nch = (n + CH - 1) / CH;
+    u32 pf[9];
+    stage_fetch(in, n, (nch - 1) * CH, pf);
+    stage_remap(sh, pf);
+    stage_put(sh.stage[(nch - 1) & 1], pf);
+    __syncthreads();
+    for (u64 ch = nch; ch-- > 0;) {
+        const u64 p0 = ch * CH;
+        if (ch > 0) stage_fetch(in, n, p0 - CH, pf);
+        const u8 *sb = (const u8 *)sh.stage[ch & 1];
+        const u64 pend = p0 + CH < n ? p0 + CH : n;
+        const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
+        u64 r = r_hi + 1;
+        while (r >= r_lo + WALK_BATCH) {
+            u32 sy[WALK_BATCH], cx[WALK_BATCH], fv[WALK_BATCH], mv[WALK_BATCH];
+            const u32 l = lane_id();
+#pragma unroll
+            for (u32 k = 0; k < WALK_BATCH; k++) {
+                const u32 t = (u32)((r - 1 - k) * NSTATES + (l & (NSTATES - 1)) - p0);
+                sy[k] = sb[4 + t];
+                cx[k] = sb[3 + t];
+            }
+            pin16(sy);
+            pin16(cx);
+#pragma unroll
+            for (u32 k = 0; k < WALK_BATCH; k++) {
+                const bool act = (r - 1 - k) * NSTATES + (l & (NSTATES - 1)) < n;
+                u32 e = l < NSTATES ? cx[k] * cols + sy[k] : base0 + sy[k];
+                e = act ? e : 0u;
+                fv[k] = sh.fc[e];
+                mv[k] = sh.mg[e];
+            }
+            pin16(fv);
+            pin16(mv);
+#pragma unroll
+            for (u32 k = 0; k < WALK_BATCH; k++) {
+                const bool act = (r - 1 - k) * NSTATES + (l & (NSTATES - 1)) < n;
+                round_step_both(x, wp1, wp0, act ? fv[k] : (TOTFREQ << 12), act ? mv[k] : 1u, trash);
+            }
+            r -= WALK_BATCH;
+        }
+        while (r > r_lo) {
+            u32 fcm, mg;
+            round_entry_both(sh, sb, p0, r - 1, n, cols, base0, fcm, mg);
+            round_step_both(x, wp1, wp0, fcm, mg, trash);
+            r--;
+        }
+        if (ch > 0) {
+            stage_remap(sh, pf);
+            __syncthreads();
+            stage_put(sh.stage[(ch - 1) & 1], pf);
+            __syncthreads();
+        }
+    }
+    x_out = x;
+    W1 = (u64)(top1 - wp1);
+    W0 = (u64)(top0 - wp0);
+}
+
 // Input dword t (bytes [4t, 4t+4)) as two aligned loads and a byte shift.
 // Both loads stay within [in, in + n) for t < bulk_dwords(n); the dword
 // A[t - 1] read for the byte before dword t (t >= 1) starts at or after the
@@ -348,7 +501,7 @@ __device__ __forceinline__ u64 bulk_dwords(u64 n) { return n >= 8 ? (n - 8) / 4 
 // fewer lanes then meet on one counter.  The rows are summed at the end.
 constexpr u32 HSUB = 4;
 static_assert(HSUB * 257 <= TAB_CAP, "sub-histograms live in sh.fc");
-__device__ void hist0(EncShared &sh, const u8 *in, u64 n)
+__device__ __forceinline__ void hist0(EncShared &sh, const u8 *in, u64 n)
 {
     const u32 l = lane_id();
     u32 *sub = sh.fc + (l % HSUB) * 257;
@@ -400,7 +553,7 @@ __device__ __forceinline__ void count_pair(EncShared &sh, u32 *gcounts, u32 cols
 }
 
 template <bool DENSE>
-__device__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols)
+__device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols)
 {
     const u32 l = lane_id();
     const u64 nd = (n + 3) / 4, nbulk = bulk_dwords(n);
@@ -493,11 +646,17 @@ __device__ __forceinline__ SizeBounds size_bounds(u64 dn, u64 up, u64 n, u64 hdr
 // both and compare exactly).
 struct BestGate {
     SizeBounds b0;
-    int decided;
+    int decided; // 2 after encode_job walked both orders (l0 set)
+    u8 *o0;      // the order-0 stream's region (j0) and its capacity
+    u64 cap0;
+    u64 l0;
 };
 
 // `have_hist`: sh.hist already holds this input's byte histogram (computed
-// by encode_best_job before either order).  `gate`: see BestGate.
+// by encode_best_job before either order).  `gate`: see BestGate; only the
+// GATED instance (encode-best's order-1 call) has the early choice and the
+// two-order walk compiled in, so the plain encode kernel carries neither.
+template <bool GATED>
 __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb_in, u64 &out_len,
                           u8 *ws, bool have_hist = false, BestGate *gate = nullptr)
 {
@@ -612,7 +771,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
                 c[i] = gcounts[ctx * 256 + s];
         }
         normalize_wave(c, f);
-        if (gate)
+        if (GATED && gate)
             for (int i = 0; i < 4; i++) acc_log_bits(c[i], f[i], bits_dn, bits_up);
         put_freqs(o + HDR0 + BITMAP + 512ull * d, f);
         const u32 tot = f[0] + f[1] + f[2] + f[3];
@@ -634,13 +793,52 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         }
         __syncthreads();
     }
-    if (gate) {
+    if (GATED && gate) {
         const SizeBounds b1 = size_bounds(bits_dn, bits_up, n, hdr);
         gate->decided = b1.hi < gate->b0.lo ? 1 : (gate->b0.hi < b1.lo ? 0 : -1);
         if (gate->decided == 0) {
             out_len = 0;
             return 0;
         }
+    }
+    if (GATED && gate && gate->decided == -1 && dense && (asz + 1) * asz + asz <= TAB_CAP) {
+        // Neither order can be ruled out: walk both at once.  The order-0
+        // table (from the byte histogram, indexed by dense id) goes behind the
+        // order-1 table; the order-0 stream's header is written to its own
+        // region as encode_job would for order 0.
+        const u32 base0 = (asz + 1) * asz;
+        u8 *o0 = gate->o0;
+        u32 c[4], f[4];
+        for (int i = 0; i < 4; i++) c[i] = sh.hist[4 * l + i];
+        normalize_wave(c, f);
+        if (l < HDR0) o0[l] = l == 0 ? (u8)0 : (u8)(n >> (8 * (l - 1)));
+        put_freqs(o0 + HDR0, f);
+        const u32 tot = f[0] + f[1] + f[2] + f[3];
+        u32 cum = wave_scan_incl(tot) - tot;
+        for (int i = 0; i < 4; i++) {
+            const u32 s = 4 * l + i;
+            if (f[i]) {
+                sh.fc[base0 + sh.colmap[s]] = (f[i] << 12) | cum;
+                sh.mg[base0 + sh.colmap[s]] = magic_of(f[i]);
+            }
+            cum += f[i];
+        }
+        __syncthreads();
+        u8 *wend0 = (u8 *)((uintptr_t)(o0 + gate->cap0) & ~(uintptr_t)1);
+        u32 x;
+        u64 W1, W0;
+        walk_both(sh, in, n, asz, base0, wend, wend0, ws + WS_TRASH, x, W1, W0);
+        if (l < NSTATES)
+            store_bytes(o + hdr - 4 * NSTATES + 4 * l, x, 4);
+        else
+            store_bytes(o0 + O0_HDR - 4 * NSTATES + 4 * (l - NSTATES), x, 4);
+        __threadfence();
+        wave_copy_fwd(o + hdr, wend - 2 * W1, 2 * W1);
+        wave_copy_fwd(o0 + O0_HDR, wend0 - 2 * W0, 2 * W0);
+        out_len = hdr + 2 * W1;
+        gate->l0 = O0_HDR + 2 * W0;
+        gate->decided = 2;
+        return 0;
     }
     if (!dense) __threadfence();
     TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u, asz}
@@ -697,20 +895,23 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
         normalize_wave(c, f);
         u64 dn = 0, up = 0;
         for (int i = 0; i < 4; i++) acc_log_bits(c[i], f[i], dn, up);
-        BestGate g{size_bounds(dn, up, jb.src_len, O0_HDR), -1};
-        err = encode_job(sh, src, dst, j1, l1, ws, true, &g);
+        BestGate g{size_bounds(dn, up, jb.src_len, O0_HDR), -1, dst + j0.dst_off, j0.dst_cap, 0};
+        err = encode_job<true>(sh, src, dst, j1, l1, ws, true, &g);
         if (err) return err;
         decided = g.decided;
-        if (decided != 1) {
+        if (decided == 2) {
+            l0 = g.l0; // both walked in one pass: the exact compare decides
+            decided = -1;
+        } else if (decided != 1) {
             __syncthreads();
-            err = encode_job(sh, src, dst, j0, l0, ws, true);
+            err = encode_job<false>(sh, src, dst, j0, l0, ws, true);
             if (err) return err;
         }
     } else {
-        err = encode_job(sh, src, dst, j0, l0, ws);
+        err = encode_job<false>(sh, src, dst, j0, l0, ws);
         if (err) return err;
         __syncthreads();
-        err = encode_job(sh, src, dst, j1, l1, ws, true);
+        err = encode_job<false>(sh, src, dst, j1, l1, ws, true);
         if (err) return err;
     }
     const bool one = decided >= 0 ? decided == 1 : (tie == 1 ? l1 <= l0 : l1 < l0);
@@ -724,10 +925,12 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
 
 } // namespace
 
+// BEST: encode-best (both orders, the smaller kept) instead of the job's order.
+template <bool BEST>
 __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, u8 *__restrict__ dst,
                                                     const fqxv_rans_job *__restrict__ jobs, u32 count,
                                                     const u32 *__restrict__ order, u32 *counter,
-                                                    i32 *status, u64 *out_len, u8 *ws, int best)
+                                                    i32 *status, u64 *out_len, u8 *ws)
 {
     __shared__ EncShared sh;
     u8 *myws = ws + (u64)blockIdx.x * WS_PER_WAVE;
@@ -739,7 +942,7 @@ __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, 
         const u32 j = order ? uni(order[w]) : w;
         const fqxv_rans_job jb = uni_job(jobs[j]);
         u64 ol = 0;
-        const int err = best ? encode_best_job(sh, src, dst, jb, ol, myws) : encode_job(sh, src, dst, jb, ol, myws);
+        const int err = BEST ? encode_best_job(sh, src, dst, jb, ol, myws) : encode_job<false>(sh, src, dst, jb, ol, myws);
         if (lane_id() == 0) {
             status[j] = err;
             out_len[j] = err ? 0 : ol;
@@ -753,7 +956,7 @@ __global__ __launch_bounds__(64) void k_rans_encode_one(const u8 *__restrict__ s
 {
     __shared__ EncShared sh;
     u64 ol = 0;
-    const int err = encode_job(sh, src, dst, jb, ol, ws);
+    const int err = encode_job<false>(sh, src, dst, jb, ol, ws);
     if (lane_id() == 0) {
         *status = err;
         *out_len = err ? 0 : ol;
@@ -770,8 +973,12 @@ hipError_t fqxv_launch_encode_batch(const u8 *src, u8 *dst, const fqxv_rans_job 
                                     u32 grid, hipStream_t st, int best)
 {
     hipLaunchKernelGGL(k_reset_counter_enc, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL(k_rans_encode, dim3(grid), dim3(64), 0, st, src, dst, jobs, count, order, counter,
-                       status, out_len, ws, best);
+    if (best)
+        hipLaunchKernelGGL(k_rans_encode<true>, dim3(grid), dim3(64), 0, st, src, dst, jobs, count, order, counter,
+                           status, out_len, ws);
+    else
+        hipLaunchKernelGGL(k_rans_encode<false>, dim3(grid), dim3(64), 0, st, src, dst, jobs, count, order,
+                           counter, status, out_len, ws);
     return hipGetLastError();
 }
 
@@ -784,5 +991,5 @@ hipError_t fqxv_launch_encode_one(const u8 *src, u8 *dst, const fqxv_rans_job &j
 
 hipError_t fqxv_encode_occupancy(int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_rans_encode, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_rans_encode<false>, 64, 0);
 }

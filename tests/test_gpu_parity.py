@@ -349,11 +349,57 @@ def test_encode_best_random_blocks_match_oracle(coder, oracle):
         assert got == oracle.encode_best(b, int(ties[j])), j
 
 
+def test_encode_best_both_orders_one_walk_edges(coder, oracle):
+    """When neither order's size bounds win, both orders are walked in one
+    pass (order-1 states in lanes 0..31, order-0 in 32..63) if both tables
+    fit the shared table (alphabets up to 41 symbols), else one after the
+    other.  Sources with 41 and 42 symbols whose order-1 gain sweeps from
+    none to large (the previous symbol repeated with probability p), at sizes
+    past the early-choice threshold with partial last rounds, alternating tie
+    rules: device == oracle for every block, and decoded back."""
+    import torch
+    rng = np.random.default_rng(4141)
+    blocks, ties = [], []
+    # (p, n): the first three are decided by the bounds (order-0), the next
+    # six are not (by the oracle's streams and the device's integer bounds:
+    # six undecided blocks per alphabet), the last picks order-1.
+    cases = [(0.0, 16385), (0.1, 70001), (0.5, 70001), (0.2, 300007), (0.22, 300007), (0.24, 300007),
+             (0.26, 300007), (0.18, 700001), (0.2, 700001), (0.22, 700001)]
+    for a in (41, 42):
+        for p, n in cases:
+            src = rng.integers(0, a, n)
+            rep = rng.random(n) < p
+            for i in np.flatnonzero(rep):
+                if i:
+                    src[i] = src[i - 1]
+            blocks.append((src + 33).astype(np.uint8).tobytes())
+            ties.append(len(blocks) % 2)
+    ties = np.array(ties, dtype=np.int32)
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    offs = np.zeros(len(blocks), dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)[:-1]
+    arena = torch.from_numpy(np.frombuffer(b"".join(blocks) + bytes(64), dtype=np.uint8).copy()).cuda()
+    r = coder.encode_best_batch(arena, offs, sizes, ties)
+    torch.cuda.synchronize()
+    assert np.all(r.status.cpu().numpy() == 0)
+    ol = r.out_len.cpu().numpy()
+    host = r.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        got = host[int(r.dst_off[j]):int(r.dst_off[j]) + int(ol[j])].tobytes()
+        assert got == oracle.encode_best(b, int(ties[j])), j
+    d = coder.decode_batch(r.dst, r.dst_off, ol, sizes)
+    torch.cuda.synchronize()
+    assert np.all(d.status.cpu().numpy() == 0)
+    out = d.dst.cpu().numpy()
+    for j, b in enumerate(blocks):
+        assert out[int(d.dst_off[j]):int(d.dst_off[j]) + len(b)].tobytes() == b, j
+
+
 def test_order1_widest_tables_on_every_wave(coder, oracle):
     """Each wave's order-1 code table starts at its own offset in the wave's
     workspace.  A batch of more widest-table blocks than there are waves --
     256 contexts (the 1 MiB table of the wide path) and 64 contexts (the
-    256 KiB table of the fast path), alternating -- has every wave, so every
+    fast path's widest rows), alternating -- has every wave, so every
     offset, build and gather a widest table: decode == input for each."""
     import torch
     coder.set_grid(0)

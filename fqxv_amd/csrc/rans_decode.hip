@@ -49,8 +49,13 @@ constexpr u32 FAST_ROUNDS = 4;                       // rounds per branch-free b
 // 16-byte row writes of consecutive lanes start on different LDS banks.
 constexpr u32 MROW = 80;
 
+// Order-1 region past the order-0 slot table: the whole workgroup stays within
+// 40 KiB, so four one-wave workgroups still share a CU's 160 KiB of LDS.
+constexpr u32 O1_EXTRA_WORDS = 1832;
+constexpr u32 TAB_WORDS = TOTFREQ + O1_EXTRA_WORDS;
+
 struct DecShared {
-    u32 tab[TOTFREQ];                                 // o0 slot table; o1: fc2[64][64]
+    u32 tab[TAB_WORDS];                               // o0 slot table; o1: fc2[K][nu], then the code rows
     u32 ring[(RING_WORDS + RING_MIRROR) / 2];         // renormalisation word ring
     union {
         u32 outst[OUT_BYTES / 4];                     // output staging (decode loop)
@@ -62,6 +67,7 @@ struct DecShared {
                                                       // (set-up: slot boundary bitmap, 128 words)
     alignas(16) u16 rowidx[TOTFREQ];                  // order-1: slot -> distinct code row
 };
+static_assert(sizeof(DecShared) <= 40960, "four decode workgroups per CU");
 
 // Words region reader (see the file comment).  Chunk c covers words
 // [256c, 256c + 256).  While the read position is in chunk c, chunks c, c+1
@@ -401,12 +407,13 @@ __device__ __forceinline__ u8 *table_base(u8 *ws)
     return ws + ((((u64)blockIdx.x * 2654435761u) & 0xffffffffu) >> 19 << 7);
 }
 
-// Fast path: K <= 64 contexts and |contexts u symbols| <= 64.
-// Workspace: t8[row * S + d] (u8) = lane(sym) for each distinct code row, S =
-// K rounded up to 16, the bytes past K zero (context 0, a valid lane).  LDS:
-// rowidx[slot] = the slot's row, fc2[d * 64 + lane(sym)] = (f-1) << 20 |
-// cum << 8 | sym, and the round's matrix mat[j][c] (rows of MROW bytes) = the
-// code context c decodes to at position j.
+// Fast path: K <= 64 contexts and |contexts u symbols| = nu <= 64.
+// t8[row * S + d] (u8) = lane(sym) for each distinct code row, S = K rounded
+// up to 16, the bytes past K zero (context 0, a valid lane): in LDS behind
+// fc2 when it fits, else in the wave's workspace.  LDS: rowidx[slot] = the
+// slot's row, fc2[d * nu + lane(sym)] = (f-1) << 20 | cum << 8 | sym, and the
+// round's matrix mat[j][c] (rows of MROW bytes) = the code context c decodes
+// to at position j.
 //
 // Per round: each lane looks up its slot's row and loads 64 bytes there
 // (issued at the end of the round before) and stores them as its matrix
@@ -422,11 +429,14 @@ __device__ __forceinline__ u8 *table_base(u8 *ws)
 // those tables overflowed L2 (hit rate 40% against 97% at one per CU) and a
 // round slowed from 555 to 810 ns.  The distinct rows number 21 (DNA), about
 // 100 (names) and about 300 (K = 41 qualities) on the test corpora.
-__device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K, u32 x,
-                              u64 hdr, u8 *ws)
+template <bool LDS_ROWS>
+__device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K,
+                                              u32 nu, u32 x, u64 hdr, u8 *ws, u32 rows_off)
 {
     const u32 l = lane_id();
-    u8 *t8 = table_base(ws);
+    // The code rows: in LDS behind fc2 when they fit (LDS_ROWS), else in the
+    // wave's workspace.
+    u8 *t8 = LDS_ROWS ? (u8 *)sh.tab + rows_off : table_base(ws);
     u32 *fc2 = sh.tab;
     // Distinct code rows.  The row of slot k (the code every context decodes
     // at k) changes only at the boundary slots o1_validate marked, so the
@@ -460,9 +470,10 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
     const u32 nrows = uni(sh.rowidx[TOTFREQ - 1]) + 1;
     // Rows are 16-byte aligned: S >= K bytes each.  A round reads 64 bytes
     // from each row start: past S they are the codes of the next rows, and
-    // the last row's over-read stays inside the workspace (the table ends at
-    // least 1 MiB before it).  The last 16 bytes of every row are zeroed
-    // first, so the bytes past K read as context 0 rather than a stale value.
+    // the last row's over-read stays inside the region (the caller leaves
+    // 64 bytes for it in LDS; in the workspace the table ends at least 1 MiB
+    // before the end).  The last 16 bytes of every row are zeroed first, so
+    // the bytes past K read as context 0 rather than a stale value.
     const u32 S = (K + 15) & ~15u;
     for (u32 i = l; i < nrows; i += 64) *(uint4 *)(t8 + i * S + (S - 16)) = make_uint4(0, 0, 0, 0);
     __builtin_amdgcn_s_waitcnt(0);
@@ -478,7 +489,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
 #pragma unroll
             for (u32 i = 0; i < 4; i++) {
                 const u32 sym = 4 * l + i, fcv = sh.fc[sym], f = fcv >> 16;
-                if (f) fc2[d * UMAX + sh.ulane[sym]] = ((f - 1) << 20) | ((fcv & 0xffff) << 8) | sym;
+                if (f) fc2[d * nu + sh.ulane[sym]] = ((f - 1) << 20) | ((fcv & 0xffff) << 8) | sym;
             }
             if (starts) {
                 // Owner of slot 64l: the largest symbol with cum <= 64l.
@@ -501,7 +512,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
             __syncthreads();
         }
     }
-    __threadfence();
+    if (!LDS_ROWS) __threadfence();
 
     WordRing wr;
     wr.init(s + hdr, end);
@@ -596,7 +607,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
                 const u32 k4 = l & 3;
                 const u32 sc = (dq >> (8 * ((k4 + 1) & 3))) & 0xffu;
                 const u32 cc = (((dq & 0xffffff00u) | (cq & 0xffu)) >> (8 * k4)) & 0xffu;
-                const u32 ent = fc2[((cc & 63) << 6) | (sc & 63)];
+                const u32 ent = fc2[(cc & 63) * nu + (sc & 63)];
                 const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
                 const bool need = lo && v < RANS_L;
                 const u64 m = ballot(need);
@@ -646,7 +657,7 @@ __device__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out
         const u64 nb = ballot(lo && l < cntr && (cv & 0xff) >= K);
         const u32 jb = nb ? (u32)__builtin_ctzll(nb) : NSTATES;
         const bool act = lo && l < cntr && l < jb;
-        const u32 ent = fc2[(cv & 0x3f) * UMAX + ((cv >> 8) & 0x3f)];
+        const u32 ent = fc2[(cv & 0x3f) * nu + ((cv >> 8) & 0x3f)];
         const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
         int err = renorm(sh, wr, x, v, act, pos, avail);
         if (err) return err;
@@ -818,7 +829,18 @@ __device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
         }
     }
     __syncthreads();
-    if (K <= UMAX && nu <= UMAX) return decode_o1_fast(sh, s, end, out, n, K, x, hdr, ws);
+    if (K <= UMAX && nu <= UMAX) {
+        // Distinct code rows (1 + the boundary slots o1_validate marked) and
+        // where they go: LDS behind fc2[K][nu] when they fit with the 64-byte
+        // over-read of the last row, else the workspace.
+        const u32 *bm = (const u32 *)sh.mat;
+        const u32 nrows = 1 + uni(wave_sum(__popc(bm[2 * l]) + __popc(bm[2 * l + 1])));
+        const u32 S = (K + 15) & ~15u;
+        const u32 rows_off = (K * nu * 4 + 15) & ~15u;
+        if (rows_off + nrows * S + 64 <= TAB_WORDS * 4)
+            return decode_o1_fast<true>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
+        return decode_o1_fast<false>(sh, s, end, out, n, K, nu, x, hdr, ws, 0);
+    }
     return decode_o1_multi(sh, s, end, out, n, K, x, hdr, ws);
 }
 

@@ -33,6 +33,12 @@ namespace {
 // context 0 when symbol 0 is absent, columns = the alphabet.  Its capacity
 // keeps the whole shared block within 20 KiB, so two waves fit per SIMD.
 constexpr u32 TAB_CAP = 1808;
+// Table entry of lanes with no position in a full walk round (lanes 32..63
+// of a one-order walk): f = 4096, which no state reaches, so the step leaves
+// x unchanged and emits nothing.  Above every table the walks index (dense
+// order-1 up to 43 x 42 = 1806 entries, both orders up to 42 x 41 + 41).
+constexpr u32 TAB_IDLE = TAB_CAP - 1;
+static_assert(43 * 42 <= TAB_IDLE && 42 * 41 + 41 <= TAB_IDLE, "idle entry above every table");
 constexpr u32 CH = 2048;                 // walk staging chunk (positions)
 constexpr u32 CH_DW = CH / 4 + 1;        // + 4 leading bytes (context of the first)
 #ifndef WALK_BATCH
@@ -301,6 +307,12 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
     gu16 *wp = wtop;
     u32 x = RANS_L; // a register for the whole walk, written back at the end
     const u64 nch = (n + CH - 1) / CH;
+    const u32 l = lane_id();
+    const bool lo = l < NSTATES;
+    if (l == 0) {
+        sh.fc[TAB_IDLE] = TOTFREQ << 12;
+        sh.mg[TAB_IDLE] = 1;
+    }
     u32 pf[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
     if (MODE == 1) stage_remap(sh, pf);
@@ -313,13 +325,22 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
         const u64 pend = p0 + CH < n ? p0 + CH : n;
         const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
         u64 r = r_hi + 1; // rounds r-1 down to r_lo remain
+        // The block's top round may be partial: it takes the exact step
+        // first, so every batch below is full (32 positions in [0, n)).
+        if (pend == n && (n % NSTATES) != 0) {
+            u32 fcm, mg;
+            round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);
+            round_step(x, wp, fcm, mg, trash);
+            r--;
+        }
         while (r >= r_lo + WALK_BATCH) {
             // Three phases, each's reads in flight together: the staged
             // bytes, then the table entries they index, then the steps.
             u32 sy[WALK_BATCH], cx[WALK_BATCH], fv[WALK_BATCH], mv[WALK_BATCH];
+            const u32 tl = (u32)(r * NSTATES - p0) + (l & (NSTATES - 1));
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
-                const u32 t = (u32)((r - 1 - k) * NSTATES + (lane_id() & (NSTATES - 1)) - p0);
+                const u32 t = tl - NSTATES * (k + 1);
                 sy[k] = sb[4 + t];
                 cx[k] = MODE == 0 ? 0u : sb[3 + t];
             }
@@ -327,18 +348,25 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
             pin16(cx);
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
-                const bool act = lane_id() < NSTATES && (r - 1 - k) * NSTATES + lane_id() < n;
                 u32 e = MODE == 0 ? sy[k] : (MODE == 1 ? cx[k] * tr.cols + sy[k] : cx[k] * 256 + sy[k]);
-                e = act ? e : 0u;
-                fv[k] = MODE == 2 ? tr.fc[e] : sh.fc[e];
-                mv[k] = MODE == 2 ? tr.mg[e] : sh.mg[e];
+                if (MODE == 2) {
+                    e = lo ? e : 0u;
+                    fv[k] = tr.fc[e];
+                    mv[k] = tr.mg[e];
+                } else {
+                    e = lo ? e : TAB_IDLE;
+                    fv[k] = sh.fc[e];
+                    mv[k] = sh.mg[e];
+                }
             }
             pin16(fv);
             pin16(mv);
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
-                const bool act = lane_id() < NSTATES && (r - 1 - k) * NSTATES + lane_id() < n;
-                round_step(x, wp, act ? fv[k] : (TOTFREQ << 12), act ? mv[k] : 1u, trash);
+                if (MODE == 2)
+                    round_step(x, wp, lo ? fv[k] : (TOTFREQ << 12), lo ? mv[k] : 1u, trash);
+                else
+                    round_step(x, wp, fv[k], mv[k], trash);
             }
             r -= WALK_BATCH;
         }
@@ -431,6 +459,7 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
     gu16 *wp1 = top1, *wp0 = top0;
     u32 x = RANS_L;
     const u64 nch = (n + CH - 1) / CH;
+    const u32 l = lane_id();
     u32 pf[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
     stage_remap(sh, pf);
@@ -443,12 +472,20 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
         const u64 pend = p0 + CH < n ? p0 + CH : n;
         const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
         u64 r = r_hi + 1;
+        // A partial top round first (exact step), then full batches only:
+        // every lane of a full round is active.
+        if (pend == n && (n % NSTATES) != 0) {
+            u32 fcm, mg;
+            round_entry_both(sh, sb, p0, r - 1, n, cols, base0, fcm, mg);
+            round_step_both(x, wp1, wp0, fcm, mg, trash);
+            r--;
+        }
         while (r >= r_lo + WALK_BATCH) {
             u32 sy[WALK_BATCH], cx[WALK_BATCH], fv[WALK_BATCH], mv[WALK_BATCH];
-            const u32 l = lane_id();
+            const u32 tl = (u32)(r * NSTATES - p0) + (l & (NSTATES - 1));
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
-                const u32 t = (u32)((r - 1 - k) * NSTATES + (l & (NSTATES - 1)) - p0);
+                const u32 t = tl - NSTATES * (k + 1);
                 sy[k] = sb[4 + t];
                 cx[k] = sb[3 + t];
             }
@@ -456,19 +493,14 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
             pin16(cx);
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
-                const bool act = (r - 1 - k) * NSTATES + (l & (NSTATES - 1)) < n;
-                u32 e = l < NSTATES ? cx[k] * cols + sy[k] : base0 + sy[k];
-                e = act ? e : 0u;
+                const u32 e = l < NSTATES ? cx[k] * cols + sy[k] : base0 + sy[k];
                 fv[k] = sh.fc[e];
                 mv[k] = sh.mg[e];
             }
             pin16(fv);
             pin16(mv);
 #pragma unroll
-            for (u32 k = 0; k < WALK_BATCH; k++) {
-                const bool act = (r - 1 - k) * NSTATES + (l & (NSTATES - 1)) < n;
-                round_step_both(x, wp1, wp0, act ? fv[k] : (TOTFREQ << 12), act ? mv[k] : 1u, trash);
-            }
+            for (u32 k = 0; k < WALK_BATCH; k++) round_step_both(x, wp1, wp0, fv[k], mv[k], trash);
             r -= WALK_BATCH;
         }
         while (r > r_lo) {

@@ -236,8 +236,8 @@ def bench_c4(args, coder, dev, oracle):
     else:
         tuned = tune.autotune(coder, "decode", enc.dst, out, jobs_np, corpus.sizes, corpus.orders,
                               use_cache=False, log=log)
-    log(f"[c4] tuned grid={tuned.grid} policy={tuned.policy}")
-    g = coder.graph("decode", enc.dst, out, jobs_np, tuned.order(coder, corpus.sizes, corpus.orders))
+    log(f"[c4] tuned grid={tuned.grid} policy={tuned.policy} staging={tuned.staging}")
+    g = coder.graph("decode", enc.dst, out, tuned.jobs(jobs_np), tuned.order(coder, corpus.sizes, corpus.orders))
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         g.launch(stream)
@@ -883,7 +883,8 @@ def main():
         "latency_floor": c4["largest_o1"],
         "ns_per_round_alone": c4["kind_rounds"],
         "percall": percall,
-        "tuner": {"grid": tuned.grid, "policy": tuned.policy, "table": tuned.table},
+        "tuner": {"grid": tuned.grid, "policy": tuned.policy, "staging": tuned.staging,
+                  "dimensions": ["grid", "policy", "staging"], "table": tuned.table},
         "encode": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s", "form": "batched, one persistent launch"},
         "encode_best": c4["encode_best"],
         "crc32c": c4["crc32c"],

@@ -23,6 +23,7 @@ ERR_ABSENT_CONTEXT = 6
 ERR_BOUND = 7
 ERR_DST_TOO_SMALL = 8
 ERR_INVALID_ARG = 9
+JOB_O1_ROWS_WORKSPACE = 1  # fqxv_rans_job.flags: order-1 code rows in the workspace, not LDS
 ERR_DEVICE = 10
 ERR_SCRATCH_LIMIT = 11
 BACKEND_GFX950 = 0

@@ -802,7 +802,8 @@ __device__ int decode_o1_multi(DecShared &sh, const u8 *s, const u8 *end, u8 *ou
     return 0;
 }
 
-__device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap, u8 *ws)
+// rows_lds: the fast path may keep its code rows in LDS (job flags).
+__device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap, u8 *ws, bool rows_lds)
 {
     const u32 l = lane_id();
     u32 K, symp;
@@ -837,7 +838,7 @@ __device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
         const u32 nrows = 1 + uni(wave_sum(__popc(bm[2 * l]) + __popc(bm[2 * l + 1])));
         const u32 S = (K + 15) & ~15u;
         const u32 rows_off = (K * nu * 4 + 15) & ~15u;
-        if (rows_off + nrows * S + 64 <= TAB_WORDS * 4)
+        if (rows_lds && rows_off + nrows * S + 64 <= TAB_WORDS * 4)
             return decode_o1_fast<true>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
         return decode_o1_fast<false>(sh, s, end, out, n, K, nu, x, hdr, ws, 0);
     }
@@ -852,7 +853,9 @@ __device__ int decode_job_core(DecShared &sh, const u8 *src, u8 *dst, const fqxv
     const u8 *end = s + jb.src_len;
     int err = 0;
     u64 n = 0;
-    if (jb.src_len < HDR0) {
+    if (jb.flags & ~FQXV_RANS_JOB_O1_ROWS_WORKSPACE) {
+        err = FQXV_RANS_ERR_INVALID_ARG;
+    } else if (jb.src_len < HDR0) {
         err = FQXV_RANS_ERR_TRUNCATED;
     } else {
         const u32 b = lane_id() < HDR0 ? s[lane_id()] : 0;
@@ -868,7 +871,8 @@ __device__ int decode_job_core(DecShared &sh, const u8 *src, u8 *dst, const fqxv
                 // wave issues first on a shared SIMD.
                 const bool crit = n >= (1u << 20);
                 if (crit) __builtin_amdgcn_s_setprio(3);
-                err = decode_o1(sh, s, end, dst + jb.dst_off, n, jb.dst_cap, ws);
+                err = decode_o1(sh, s, end, dst + jb.dst_off, n, jb.dst_cap, ws,
+                                (jb.flags & FQXV_RANS_JOB_O1_ROWS_WORKSPACE) == 0);
                 if (crit) __builtin_amdgcn_s_setprio(0);
             } else
                 err = FQXV_RANS_ERR_UNKNOWN_ORDER;

@@ -192,7 +192,7 @@ class Coder:
         return ctypes.c_void_p(stream.cuda_stream)
 
     @staticmethod
-    def jobs_array(src_off, src_len, dst_off, dst_cap, max_len=None, order=None) -> np.ndarray:
+    def jobs_array(src_off, src_len, dst_off, dst_cap, max_len=None, order=None, flags=0) -> np.ndarray:
         k = len(src_off)
         a = np.zeros(k, dtype=np.dtype(_lib.JOB_DTYPE_FIELDS))
         a["src_off"] = src_off
@@ -201,6 +201,7 @@ class Coder:
         a["dst_cap"] = dst_cap
         a["max_len"] = (np.uint64(2**64 - 1) if max_len is None else max_len)
         a["order"] = 0 if order is None else order
+        a["flags"] = flags
         return a
 
     def _run_batch(self, fn, src, dst, jobs_np, order_np, stream):

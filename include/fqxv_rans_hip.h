@@ -87,8 +87,14 @@ typedef struct fqxv_rans_job {
     uint64_t dst_cap;
     uint64_t max_len;
     int32_t order;
-    uint32_t flags; /* reserved, must be 0 */
+    uint32_t flags; /* FQXV_RANS_JOB_* bits (decode); other bits must be 0 */
 } fqxv_rans_job;
+
+/* Decode: keep an order-1 block's distinct code rows in the wave's device
+ * workspace (read through L1/L2) even when they fit in LDS, where they go
+ * otherwise.  A staging choice for tuning (fqxv_amd.tune); the output is
+ * the same either way.  No counterpart in the reference (a GPU placement). */
+#define FQXV_RANS_JOB_O1_ROWS_WORKSPACE 1u
 
 /* Exact message of a return code (the reference's Malformed strings). */
 const char *fqxv_rans_strerror(int code);

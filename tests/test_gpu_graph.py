@@ -142,12 +142,13 @@ def test_autotune_picks_a_measured_candidate_and_caches(coder, tmp_path, monkeyp
     jobs = coder.jobs_array(enc.dst_off, ol, dst_off, sizes)
     try:
         t = tune.autotune(coder, "decode", enc.dst, dst, jobs, sizes, orders, grids=[40, 8], reps=1)
-        assert (t.grid, t.policy, t.ms) in [tuple(r) for r in t.table]
-        assert len(t.table) == 2 * len(tune.POLICIES)
+        assert (t.grid, t.policy, t.staging, t.ms) in [tuple(r) for r in t.table]
+        assert len(t.table) == 2 * len(tune.POLICIES) * len(tune.STAGINGS)
+        assert {r[2] for r in t.table} == set(tune.STAGINGS)
         assert (tmp_path / "tune.json").exists()
         again = tune.autotune(coder, "decode", enc.dst, dst, jobs, sizes, orders, grids=[40, 8], reps=1)
-        assert (again.grid, again.policy) == (t.grid, t.policy)
-        g = coder.graph("decode", enc.dst, dst, jobs, t.order(coder, sizes, orders))
+        assert (again.grid, again.policy, again.staging) == (t.grid, t.policy, t.staging)
+        g = coder.graph("decode", enc.dst, dst, t.jobs(jobs), t.order(coder, sizes, orders))
         dst.zero_()
         g.launch()
         torch.cuda.synchronize()

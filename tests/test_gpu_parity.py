@@ -395,6 +395,49 @@ def test_encode_best_both_orders_one_walk_edges(coder, oracle):
         assert out[int(d.dst_off[j]):int(d.dst_off[j]) + len(b)].tobytes() == b, j
 
 
+def test_order1_row_staging_flag(coder, oracle):
+    """An order-1 block decodes to the same bytes with its code rows in LDS
+    (flags 0; the rows fit for these alphabets) and in the workspace
+    (FQXV_RANS_JOB_O1_ROWS_WORKSPACE), equal to the oracle; a job with an
+    unknown flag bit fails with the invalid-argument code and writes nothing."""
+    import torch
+    from fqxv_amd import _lib, synth
+    blocks = [synth.fill(k, n, 900 + i).tobytes() for i, (k, n) in
+              enumerate([(synth.K_MARKOV, 300001), (synth.K_DNA, 70000), (synth.K_NAMES, 100003)])]
+    blocks.append(rng_bytes(907, 50000, 64))  # 64 contexts: rows past the LDS region
+    streams = [oracle.encode(b, 1) for b in blocks]
+    src = np.frombuffer(b"".join(streams) + bytes(64), dtype=np.uint8)
+    s_len = np.array([len(x) for x in streams], dtype=np.uint64)
+    s_off = np.zeros(len(streams), dtype=np.uint64)
+    s_off[1:] = np.cumsum(s_len)[:-1]
+    sizes = np.array([len(b) for b in blocks], dtype=np.uint64)
+    d_off = np.zeros(len(blocks), dtype=np.uint64)
+    d_off[1:] = np.cumsum(sizes)[:-1]
+    dsrc = torch.from_numpy(src.copy()).cuda()
+    outs = []
+    for flags in (0, _lib.JOB_O1_ROWS_WORKSPACE):
+        jobs = coder.jobs_array(s_off, s_len, d_off, sizes, flags=flags)
+        dst = torch.full((int(sizes.sum()) + 64,), 0xEE, dtype=torch.uint8, device="cuda")
+        g = coder.graph("decode", dsrc, dst, jobs)
+        g.launch()
+        torch.cuda.synchronize()
+        assert np.all(g.status.cpu().numpy() == 0), flags
+        g.close()
+        host = dst.cpu().numpy()
+        for j, b in enumerate(blocks):
+            assert host[int(d_off[j]):int(d_off[j]) + len(b)].tobytes() == b, (flags, j)
+        outs.append(host)
+    assert np.array_equal(outs[0], outs[1])
+    jobs = coder.jobs_array(s_off[:1], s_len[:1], d_off[:1], sizes[:1], flags=2)
+    dst = torch.full((int(sizes[0]) + 64,), 0xEE, dtype=torch.uint8, device="cuda")
+    g = coder.graph("decode", dsrc, dst, jobs)
+    g.launch()
+    torch.cuda.synchronize()
+    assert int(g.status.cpu()[0]) == _lib.ERR_INVALID_ARG
+    assert bool((dst == 0xEE).all())
+    g.close()
+
+
 def test_order1_widest_tables_on_every_wave(coder, oracle):
     """Each wave's order-1 code table starts at its own offset in the wave's
     workspace.  A batch of more widest-table blocks than there are waves --

@@ -47,7 +47,7 @@ def test_mixed_10k_full_tuned_graph_into_poisoned_buffer(coder, oracle):
     jobs = coder.jobs_array(enc.dst_off, comp_len, dst_off, corpus.sizes)
     tuned = tune.autotune(coder, "decode", enc.dst, dst, jobs, corpus.sizes, corpus.orders,
                           policies=("lpt-cost", "lpt-size"), use_cache=False)
-    g = coder.graph("decode", enc.dst, dst, jobs, tuned.order(coder, corpus.sizes, corpus.orders))
+    g = coder.graph("decode", enc.dst, dst, tuned.jobs(jobs), tuned.order(coder, corpus.sizes, corpus.orders))
     try:
         for rep in range(2):
             dst.fill_(SENTINEL)

@@ -263,9 +263,13 @@ __device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &t
 // lowest word emitted so far; the round's k words go to [wp - k, wp), the
 // emitting lane of rank r (lanes below it that emit) at wp - k + r, so the
 // highest lane's word comes first in the reference's reversed order.  Every
-// lane stores: a lane with no word writes its own slot of `trash` (64 u16 in
-// the wave's workspace), so the step has no branch and the state chain never
-// waits on an exec-mask change.
+// state lane stores: a lane with no word writes its own slot of `trash` (64
+// u16 in the wave's workspace), so the step has no branch and the state chain
+// never waits on an exec-mask change.  Lanes 32..63 (no state) do not store: a
+// fixed exec mask, not a per-step one, which halves the store addresses the
+// memory path handles per step (8% on the 10,000-block batch, where the
+// stores of many waves compete; masking every lane without a word instead
+// was slower, the mask then waits on each step's compare).
 typedef __attribute__((address_space(1))) u8 gu8;
 typedef __attribute__((address_space(1))) u16 gu16;
 __device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg, gu16 *trash)
@@ -284,7 +288,7 @@ __device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg, g
     wp -= (u32)__popcll(m);
     const u64 aw = (u64)(uintptr_t)(wp + __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
     const u64 at = (u64)(uintptr_t)(trash + lane_id());
-    *(gu16 *)(uintptr_t)(at ^ ((aw ^ at) & (0ull - (u64)nb))) = (u16)x;
+    if (lane_id() < NSTATES) *(gu16 *)(uintptr_t)(at ^ ((aw ^ at) & (0ull - (u64)nb))) = (u16)x;
     u32 q = __umulhi(v, mg);
     u32 rr = v - q * f;
     if (rr >= f) {

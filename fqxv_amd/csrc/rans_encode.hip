@@ -58,10 +58,8 @@ struct EncShared {
 static_assert(sizeof(EncShared) <= 20480, "two one-wave workgroups per SIMD need <= 20 KiB of LDS each");
 
 // Workspace of one wave (WS_PER_WAVE bytes): order-1 counts, f << 12 | cum
-// and magics for alphabets past the dense table (256 KiB each), then the
-// 128-byte line the walk's non-emitting lanes store into.
-constexpr u64 WS_TRASH = 3ull * 256 * 256 * 4;
-static_assert(WS_TRASH + 128 <= WS_PER_WAVE, "encode workspace");
+// and magics for alphabets past the dense table (256 KiB each).
+static_assert(3ull * 256 * 256 * 4 <= WS_PER_WAVE, "encode workspace");
 
 // Alphabet sizes whose dense table fits: (a + 1) * a entries.
 __device__ __forceinline__ bool dense_fits(u32 a) { return (a + 1) * a <= TAB_CAP; }
@@ -262,17 +260,27 @@ __device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &t
 // descending position order within the round).  `wp` (wave-uniform) is the
 // lowest word emitted so far; the round's k words go to [wp - k, wp), the
 // emitting lane of rank r (lanes below it that emit) at wp - k + r, so the
-// highest lane's word comes first in the reference's reversed order.  Every
-// state lane stores: a lane with no word writes its own slot of `trash` (64
-// u16 in the wave's workspace), so the step has no branch and the state chain
-// never waits on an exec-mask change.  Lanes 32..63 (no state) do not store: a
-// fixed exec mask, not a per-step one, which halves the store addresses the
-// memory path handles per step (8% on the 10,000-block batch, where the
-// stores of many waves compete; masking every lane without a word instead
-// was slower, the mask then waits on each step's compare).
+// highest lane's word comes first in the reference's reversed order.  Only
+// the emitting lanes store: one asm block sets exec to the step's ballot
+// around the store and restores it, so the step has no branch (a branch per
+// step splits a batch into blocks the scheduler cannot interleave) and the
+// memory path sees the few real words, not a store from every lane (a
+// store from every lane, the others into a trash line, was 14% slower on
+// the 10,000-block batch, where the stores of many waves compete).
+__device__ __forceinline__ void store_words(u64 m, u64 addr, u32 x)
+{
+    u64 saved;
+    asm volatile("s_mov_b64 %0, exec\n\t"
+                 "s_mov_b64 exec, %1\n\t"
+                 "global_store_short %2, %3, off\n\t"
+                 "s_mov_b64 exec, %0"
+                 : "=&s"(saved)
+                 : "s"(m), "v"(addr), "v"(x)
+                 : "memory");
+}
 typedef __attribute__((address_space(1))) u8 gu8;
 typedef __attribute__((address_space(1))) u16 gu16;
-__device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg, gu16 *trash)
+__device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg)
 {
     const u32 f = fcm >> 12, cum = fcm & 0xfff;
     // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73); inactive
@@ -286,9 +294,7 @@ __device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg, g
     const u32 v = x >> (nb << 4);
     const u64 m = ballot(need);
     wp -= (u32)__popcll(m);
-    const u64 aw = (u64)(uintptr_t)(wp + __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
-    const u64 at = (u64)(uintptr_t)(trash + lane_id());
-    if (lane_id() < NSTATES) *(gu16 *)(uintptr_t)(at ^ ((aw ^ at) & (0ull - (u64)nb))) = (u16)x;
+    store_words(m, (u64)(uintptr_t)(wp + __builtin_amdgcn_mbcnt_lo((u32)m, 0u)), x);
     u32 q = __umulhi(v, mg);
     u32 rr = v - q * f;
     if (rr >= f) {
@@ -303,10 +309,8 @@ __device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg, g
 // read before their state updates, so the LDS latency is paid once per batch
 // on the state chain.
 template <u32 MODE>
-__device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend,
-                                     u8 *trash_generic, u32 &x_out)
+__device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u32 &x_out)
 {
-    gu16 *const trash = (gu16 *)trash_generic;
     gu16 *const wtop = (gu16 *)wend;
     gu16 *wp = wtop;
     u32 x = RANS_L; // a register for the whole walk, written back at the end
@@ -334,7 +338,7 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
         if (pend == n && (n % NSTATES) != 0) {
             u32 fcm, mg;
             round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);
-            round_step(x, wp, fcm, mg, trash);
+            round_step(x, wp, fcm, mg);
             r--;
         }
         while (r >= r_lo + WALK_BATCH) {
@@ -368,16 +372,16 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
                 if (MODE == 2)
-                    round_step(x, wp, lo ? fv[k] : (TOTFREQ << 12), lo ? mv[k] : 1u, trash);
+                    round_step(x, wp, lo ? fv[k] : (TOTFREQ << 12), lo ? mv[k] : 1u);
                 else
-                    round_step(x, wp, fv[k], mv[k], trash);
+                    round_step(x, wp, fv[k], mv[k]);
             }
             r -= WALK_BATCH;
         }
         while (r > r_lo) {
             u32 fcm, mg;
             round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);
-            round_step(x, wp, fcm, mg, trash);
+            round_step(x, wp, fcm, mg);
             r--;
         }
         if (ch > 0) {
@@ -423,7 +427,7 @@ __device__ __forceinline__ void round_entry_both(const EncShared &sh, const u8 *
     mg = act ? mv : 1u;
 }
 
-__device__ __forceinline__ void round_step_both(u32 &x, gu16 *&wp1, gu16 *&wp0, u32 fcm, u32 mg, gu16 *trash)
+__device__ __forceinline__ void round_step_both(u32 &x, gu16 *&wp1, gu16 *&wp0, u32 fcm, u32 mg)
 {
     const u32 l = lane_id();
     const u32 f = fcm >> 12, cum = fcm & 0xfff;
@@ -440,9 +444,7 @@ __device__ __forceinline__ void round_step_both(u32 &x, gu16 *&wp1, gu16 *&wp0, 
     const u32 hi = l >> 5;
     const u32 rank = below - (k1 & (0u - hi));
     const u64 w1 = (u64)(uintptr_t)wp1, w0 = (u64)(uintptr_t)wp0;
-    const u64 aw = (w1 ^ ((w1 ^ w0) & (0ull - (u64)hi))) + 2ull * rank;
-    const u64 at = (u64)(uintptr_t)(trash + l);
-    *(gu16 *)(uintptr_t)(at ^ ((aw ^ at) & (0ull - (u64)nb))) = (u16)x;
+    store_words(m, (w1 ^ ((w1 ^ w0) & (0ull - (u64)hi))) + 2ull * rank, x);
     u32 q = __umulhi(v, mg);
     u32 rr = v - q * f;
     if (rr >= f) {
@@ -455,10 +457,9 @@ __device__ __forceinline__ void round_step_both(u32 &x, gu16 *&wp1, gu16 *&wp0, 
 // Returns the order-1 and order-0 word counts; the words end at wend1 and
 // wend0 (2-byte aligned); x_out: order-1 states in lanes 0..31, order-0 in
 // lanes 32..63.
-__device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u32 cols, u32 base0, u8 *wend1, u8 *wend0,
-                          u8 *trash_generic, u32 &x_out, u64 &W1, u64 &W0)
+__device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u32 cols, u32 base0, u8 *wend1,
+                                          u8 *wend0, u32 &x_out, u64 &W1, u64 &W0)
 {
-    gu16 *const trash = (gu16 *)trash_generic;
     gu16 *const top1 = (gu16 *)wend1, *const top0 = (gu16 *)wend0;
     gu16 *wp1 = top1, *wp0 = top0;
     u32 x = RANS_L;
@@ -481,7 +482,7 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
         if (pend == n && (n % NSTATES) != 0) {
             u32 fcm, mg;
             round_entry_both(sh, sb, p0, r - 1, n, cols, base0, fcm, mg);
-            round_step_both(x, wp1, wp0, fcm, mg, trash);
+            round_step_both(x, wp1, wp0, fcm, mg);
             r--;
         }
         while (r >= r_lo + WALK_BATCH) {
@@ -504,13 +505,13 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
             pin16(fv);
             pin16(mv);
 #pragma unroll
-            for (u32 k = 0; k < WALK_BATCH; k++) round_step_both(x, wp1, wp0, fv[k], mv[k], trash);
+            for (u32 k = 0; k < WALK_BATCH; k++) round_step_both(x, wp1, wp0, fv[k], mv[k]);
             r -= WALK_BATCH;
         }
         while (r > r_lo) {
             u32 fcm, mg;
             round_entry_both(sh, sb, p0, r - 1, n, cols, base0, fcm, mg);
-            round_step_both(x, wp1, wp0, fcm, mg, trash);
+            round_step_both(x, wp1, wp0, fcm, mg);
             r--;
         }
         if (ch > 0) {
@@ -732,7 +733,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         __syncthreads();
         TabRef tr{sh.fc, sh.mg, sh.rowmap, sh.colmap, 0, 0};
         u32 x;
-        const u64 W = walk<0>(sh, in, n, tr, wend, ws + WS_TRASH, x);
+        const u64 W = walk<0>(sh, in, n, tr, wend, x);
         finish(o, O0_HDR, wend, W, x);
         out_len = O0_HDR + 2 * W;
         return 0;
@@ -863,7 +864,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         u8 *wend0 = (u8 *)((uintptr_t)(o0 + gate->cap0) & ~(uintptr_t)1);
         u32 x;
         u64 W1, W0;
-        walk_both(sh, in, n, asz, base0, wend, wend0, ws + WS_TRASH, x, W1, W0);
+        walk_both(sh, in, n, asz, base0, wend, wend0, x, W1, W0);
         if (l < NSTATES)
             store_bytes(o + hdr - 4 * NSTATES + 4 * l, x, 4);
         else
@@ -880,8 +881,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u, asz}
                       : TabRef{gfc, gmg, sh.rowmap, sh.colmap, 2u, 0};
     u32 x;
-    const u64 W = dense ? walk<1>(sh, in, n, tr, wend, ws + WS_TRASH, x)
-                        : walk<2>(sh, in, n, tr, wend, ws + WS_TRASH, x);
+    const u64 W = dense ? walk<1>(sh, in, n, tr, wend, x) : walk<2>(sh, in, n, tr, wend, x);
     finish(o, hdr, wend, W, x);
     out_len = hdr + 2 * W;
     return 0;

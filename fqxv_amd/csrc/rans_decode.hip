@@ -40,7 +40,7 @@ namespace {
 constexpr u32 RING_CHUNK_WORDS = 256;                // 512 bytes per chunk
 constexpr u32 RING_CHUNKS = 4;
 constexpr u32 RING_WORDS = RING_CHUNK_WORDS * RING_CHUNKS;
-constexpr u32 RING_MIRROR = 32;                      // words mirrored past the end
+constexpr u32 RING_MIRROR = 64;                      // words mirrored past the end
 constexpr u32 OUT_ROUNDS = 32;                       // rounds per output flush
 constexpr u32 OUT_BYTES = OUT_ROUNDS * NSTATES;      // 1 KiB
 constexpr u32 UMAX = 64;                             // order-1 fast path lane space
@@ -51,7 +51,7 @@ constexpr u32 MROW = 80;
 
 // Order-1 region past the order-0 slot table: the whole workgroup stays within
 // 40 KiB, so four one-wave workgroups still share a CU's 160 KiB of LDS.
-constexpr u32 O1_EXTRA_WORDS = 1832;
+constexpr u32 O1_EXTRA_WORDS = 1824;
 constexpr u32 TAB_WORDS = TOTFREQ + O1_EXTRA_WORDS;
 
 struct DecShared {
@@ -71,9 +71,14 @@ static_assert(sizeof(DecShared) <= 40960, "four decode workgroups per CU");
 
 // Words region reader (see the file comment).  Chunk c covers words
 // [256c, 256c + 256).  While the read position is in chunk c, chunks c, c+1
-// and c+2 are resident; entering chunk c+1 loads chunk c+3.  The refill is
-// synchronous: keeping its loads in flight across rounds measured slower,
-// because later waits on unrelated loads then also wait for them.
+// and c+2 are resident.  Synchronous form: entering chunk c loads chunk c+2
+// and stores it into the ring at once.  Asynchronous form: entering chunk c
+// stores chunk c+2 (whose loads were issued on entering chunk c-1) and
+// issues the loads of chunk c+3, so a chunk's HBM latency is spread over a
+// whole chunk of rounds (a synchronous refill cost ~0.5 us every 27 rounds of
+// 4.7-bit data).  Loops that wait on other global loads (order-1 rows in the
+// workspace) use the synchronous form: a later vmcnt wait would also wait
+// for the refill's loads.
 struct WordRing {
     const u32 *A;  // 4-byte-aligned address at or below the words region
     u32 ph;        // byte offset of the region within A[0]
@@ -108,6 +113,7 @@ struct WordRing {
         ring[slot + 64 + l] = v1;
         if (slot == 0 && l < RING_MIRROR / 2) ring[RING_WORDS / 2 + l] = v0;
     }
+    template <bool ASYNC>
     __device__ void reset(u32 *ring, u64 c0)
     {
         chunk = c0;
@@ -115,14 +121,22 @@ struct WordRing {
             fetch(c);
             put(ring, c);
         }
+        if (ASYNC) fetch(c0 + 3);
         __syncthreads();
     }
+    // Called after every step of at most RING_CHUNK_WORDS words.
+    template <bool ASYNC>
     __device__ __forceinline__ void advance(u32 *ring, u64 pos)
     {
         if ((pos / RING_CHUNK_WORDS) != chunk) {
             chunk++;
-            fetch(chunk + 2);
-            put(ring, chunk + 2);
+            if (ASYNC) {
+                put(ring, chunk + 2);
+                fetch(chunk + 3);
+            } else {
+                fetch(chunk + 2);
+                put(ring, chunk + 2);
+            }
         }
     }
     // Word pos + rank, rank < 32.
@@ -239,6 +253,7 @@ __device__ int slow_round(u32 &v, u32 act_mask_lo, u64 &pos, u64 avail, const Wo
 
 // Renormalise the active lanes' v and commit them to x.  Returns 0 or the
 // truncation error.  Words are taken in lane order (scalar.rs:171-176).
+template <bool ASYNC>
 __device__ __forceinline__ int renorm(DecShared &sh, WordRing &wr, u32 &x, u32 v, bool act, u64 &pos,
                                       u64 avail)
 {
@@ -246,7 +261,7 @@ __device__ __forceinline__ int renorm(DecShared &sh, WordRing &wr, u32 &x, u32 v
         int err = slow_round(v, (u32)ballot(act), pos, avail, wr);
         if (err) return err;
         if (act) x = v;
-        wr.reset(sh.ring, pos / RING_CHUNK_WORDS);
+        wr.reset<ASYNC>(sh.ring, pos / RING_CHUNK_WORDS);
         return 0;
     }
     const bool need = act && v < RANS_L;
@@ -256,11 +271,15 @@ __device__ __forceinline__ int renorm(DecShared &sh, WordRing &wr, u32 &x, u32 v
     const u32 w = wr.word(sh.ring, pos, rank_below(m));
     if (act) x = need ? ((v << 16) | w) : v;
     pos += k;
-    wr.advance(sh.ring, pos);
+    wr.advance<ASYNC>(sh.ring, pos);
     return 0;
 }
 
 // ---------------------------------------------------------------- order-0
+// Rounds per branch-free order-0 block: one block consumes at most 256 words
+// (one ring chunk), so the ring's refill keeps up with a block at a time.
+constexpr u32 O0_BLOCK = 8;
+
 __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap)
 {
     const u32 l = lane_id();
@@ -279,7 +298,7 @@ __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     WordRing wr;
     wr.init(s + O0_HDR, end);
     const u64 avail = (len - O0_HDR) / 2;
-    wr.reset(sh.ring, 0);
+    wr.reset<true>(sh.ring, 0);
 
     const bool lo = l < NSTATES;
     const u64 full = n / NSTATES;
@@ -288,51 +307,97 @@ __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     u32 row = 0;
     u8 *o = out;
     u64 r = 0;
+    const u16 *ring16 = (const u16 *)sh.ring;
     while (r < full) {
-        if (r + FAST_ROUNDS <= full && pos + FAST_ROUNDS * NSTATES <= avail && (row % FAST_ROUNDS) == 0) {
-            // Fast block: FAST_ROUNDS rounds with no branch inside.  No
-            // truncation is possible (enough words remain), the ring holds
-            // every word the block can reach (at most 128 words, the ring
-            // keeps 512 ahead of the chunk start), and a state that would
-            // need a second word (only v == 0 does, and only then is the
-            // renormalised state below 2^16) sends the whole block back to
-            // the exact per-round path below.
-            const u32 x0 = x;
-            const u64 pos0 = pos;
-            u64 bad = 0;
+        if ((row % O0_BLOCK) == 0) {
+            // Branch-free blocks of O0_BLOCK rounds, as many as the rounds
+            // left and the words left allow (counted once, in 32 bits: a
+            // block consumes at most 32 * O0_BLOCK words).  No truncation is
+            // possible inside, the ring holds every word a block can reach
+            // (at most 256 consumed and 64 candidates past the last round's
+            // start; the ring keeps 512 ahead of the chunk start), and a
+            // state that would need a second word (only v == 0 does, and
+            // only then is the renormalised state below 2^16) sends the
+            // block back to the exact per-round path below.
+            //
+            // Pipeline: p is the slot entry of this lane's state; lane i
+            // holds the candidate word cw = word[cb + i] and its slot entry
+            // cp (a state renormalised with word w lands on slot w & 0xfff).
+            // A round's renormalising lanes take their word and next entry
+            // from the candidates with ds_bpermute at (pos - cb) + rank, so
+            // after the ballot one LDS-crossbar trip stands between a round
+            // and the next; the word read and its table lookup happen a
+            // round earlier, from the position the round starts at (a round
+            // consumes at most 32 words, so the next round's words lie in
+            // [pos, pos + 64)).  Lanes that do not renormalise look up
+            // v & 0xfff while the ballot resolves.  Lanes 32..63 run the
+            // same steps on a meaningless state: table indices stay in
+            // range, their bits are masked out of every count and their
+            // stores go to `dense` (unused by order-0), so the round needs
+            // no lane masks.
+            const u64 bw = (avail - pos) / (NSTATES * O0_BLOCK), br = (full - r) / O0_BLOCK;
+            u32 nblk = (u32)((bw < br ? bw : br) < 0x7fffffffull ? (bw < br ? bw : br) : 0x7fffffffull);
+            if (nblk) {
+                u32 p = sh.tab[x & (TOTFREQ - 1)];
+                u64 cb = pos;
+                u32 cw = ring16[(u32)(pos % RING_WORDS) + l];
+                u32 cp = sh.tab[cw & (TOTFREQ - 1)];
+                for (; nblk; nblk--) {
+                    const u32 x0 = x;
+                    const u64 pos0 = pos;
+                    u32 bad = 0;
 #pragma unroll
-            for (u32 t = 0; t < FAST_ROUNDS; t++) {
-                const u32 p = sh.tab[x & (TOTFREQ - 1)];
-                const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
-                const bool need = lo && v < RANS_L;
-                const u64 m = ballot(need);
-                const u32 w = ((const u16 *)sh.ring)[(u32)(pos % RING_WORDS) +
-                                                     __builtin_amdgcn_mbcnt_lo((u32)m, 0u)];
-                if (lo) x = need ? ((v << 16) | w) : v;
-                pos += (u32)__popcll(m);
-                if (lo) stage_byte(sh, row + t, p);
-                bad |= ballot(lo && x < RANS_L);
-            }
-            if (bad) {
-                x = x0;
-                pos = pos0;
-            } else {
-                r += FAST_ROUNDS;
-                row += FAST_ROUNDS;
-                wr.advance(sh.ring, pos);
-                if (row == OUT_ROUNDS) {
-                    flush(sh, o, OUT_BYTES);
-                    o += OUT_BYTES;
-                    row = 0;
+                    for (u32 t = 0; t < O0_BLOCK; t++) {
+                        // The next round's candidate words are read first;
+                        // their table entries are looked up after this
+                        // round's own reads, when they have arrived.
+                        u32 nw = ring16[(u32)(pos % RING_WORDS) + l];
+                        // Pin the schedule (the compiler otherwise waits on
+                        // the word read at once: one LDS round trip per
+                        // round).
+                        asm volatile("" : "+v"(x), "+v"(p)::"memory");
+                        const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
+                        const bool need = v < RANS_L;
+                        const u32 m = (u32)ballot(need);
+                        const u32 ia = ((u32)(pos - cb) + __builtin_amdgcn_mbcnt_lo(m, 0u)) << 2;
+                        u32 wn = (u32)__builtin_amdgcn_ds_bpermute((int)ia, (int)cw);
+                        u32 pn = (u32)__builtin_amdgcn_ds_bpermute((int)ia, (int)cp);
+                        u32 pv = sh.tab[v & (TOTFREQ - 1)];
+                        asm volatile("" : "+v"(nw)::"memory");
+                        const u32 np = sh.tab[nw & (TOTFREQ - 1)];
+                        asm volatile("" : "+v"(wn), "+v"(pn), "+v"(pv)::"memory");
+                        *(lo ? (u8 *)sh.outst + (row + t) * NSTATES + l : (u8 *)sh.dense + (l - NSTATES)) = (u8)p;
+                        x = need ? ((v << 16) | wn) : v;
+                        p = need ? pn : pv;
+                        cb = pos;
+                        cw = nw;
+                        cp = np;
+                        pos += (u32)__popc(m);
+                        bad |= (u32)ballot(x < RANS_L);
+                    }
+                    if (bad) {
+                        x = x0;
+                        pos = pos0;
+                        break;
+                    }
+                    r += O0_BLOCK;
+                    row += O0_BLOCK;
+                    wr.advance<true>(sh.ring, pos);
+                    if (row == OUT_ROUNDS) {
+                        flush(sh, o, OUT_BYTES);
+                        o += OUT_BYTES;
+                        row = 0;
+                    }
                 }
-                continue;
+                if (r >= full) break;
             }
         }
-        // Exact per-round path: truncation, multi-word states, ring refill.
-        const u32 p = sh.tab[x & (TOTFREQ - 1)];
-        const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
-        if ((err = renorm(sh, wr, x, v, lo, pos, avail))) return err;
-        if (lo) stage_byte(sh, row, p);
+        // Exact per-round path: truncation, multi-word states, a block that
+        // met a second word, rounds before a block boundary.
+        const u32 pe = sh.tab[x & (TOTFREQ - 1)];
+        const u32 v = ((pe >> 20) + 1) * (x >> SCALE_BITS) + ((pe >> 8) & 0xfff);
+        if ((err = renorm<true>(sh, wr, x, v, lo, pos, avail))) return err;
+        if (lo) stage_byte(sh, row, pe);
         r++;
         if (++row == OUT_ROUNDS) {
             flush(sh, o, OUT_BYTES);
@@ -342,10 +407,10 @@ __device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
     }
     if (rem) {
         const bool act = l < rem;
-        const u32 p = sh.tab[x & (TOTFREQ - 1)];
-        const u32 v = ((p >> 20) + 1) * (x >> SCALE_BITS) + ((p >> 8) & 0xfff);
-        if ((err = renorm(sh, wr, x, v, act, pos, avail))) return err;
-        if (act) stage_byte(sh, row, p);
+        const u32 pe = sh.tab[x & (TOTFREQ - 1)];
+        const u32 v = ((pe >> 20) + 1) * (x >> SCALE_BITS) + ((pe >> 8) & 0xfff);
+        if ((err = renorm<true>(sh, wr, x, v, act, pos, avail))) return err;
+        if (act) stage_byte(sh, row, pe);
     }
     flush(sh, o, row * NSTATES + rem);
     return 0;
@@ -517,7 +582,7 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
     WordRing wr;
     wr.init(s + hdr, end);
     const u64 avail = ((u64)(end - s) - hdr) / 2;
-    wr.reset(sh.ring, 0);
+    wr.reset<LDS_ROWS>(sh.ring, 0);
 
     u32 prev = sh.dense[0]; // context of symbol 0 is 0 (scalar.rs:203)
     const bool lo = l < NSTATES;
@@ -621,7 +686,7 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
             if (ballot(bad != 0) == 0 && pos <= avail) {
                 r += FAST_ROUNDS;
                 row += FAST_ROUNDS;
-                wr.advance(sh.ring, pos);
+                wr.advance<LDS_ROWS>(sh.ring, pos);
                 if (row == OUT_ROUNDS) {
                     flush(sh, o, OUT_BYTES);
                     o += OUT_BYTES;
@@ -659,7 +724,7 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
         const bool act = lo && l < cntr && l < jb;
         const u32 ent = fc2[(cv & 0x3f) * nu + ((cv >> 8) & 0x3f)];
         const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
-        int err = renorm(sh, wr, x, v, act, pos, avail);
+        int err = renorm<LDS_ROWS>(sh, wr, x, v, act, pos, avail);
         if (err) return err;
         if (act) stage_byte(sh, row, ent);
         if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
@@ -711,7 +776,7 @@ __device__ int decode_o1_multi(DecShared &sh, const u8 *s, const u8 *end, u8 *ou
     WordRing wr;
     wr.init(s + hdr, end);
     const u64 avail = ((u64)(end - s) - hdr) / 2;
-    wr.reset(sh.ring, 0);
+    wr.reset<false>(sh.ring, 0);
 
     u32 prev = sh.dense[0]; // context of symbol 0 is 0 (scalar.rs:203)
     const bool lo = l < NSTATES;
@@ -756,7 +821,7 @@ __device__ int decode_o1_multi(DecShared &sh, const u8 *s, const u8 *end, u8 *ou
             if (ballot(lo && (x < RANS_L || (ctxv & 0xff) >= K)) == 0) {
                 prev = c;
                 r++;
-                wr.advance(sh.ring, pos);
+                wr.advance<false>(sh.ring, pos);
                 if (++row == OUT_ROUNDS) {
                     flush(sh, o, OUT_BYTES);
                     o += OUT_BYTES;
@@ -785,7 +850,7 @@ __device__ int decode_o1_multi(DecShared &sh, const u8 *s, const u8 *end, u8 *ou
         const bool act = lo && l < cntr && l < jb;
         const u32 ent = act ? fcw[(cv & 0xff) * 256 + ((cv >> 8) & 0xff)] : 0u;
         const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
-        int err = renorm(sh, wr, x, v, act, pos, avail);
+        int err = renorm<false>(sh, wr, x, v, act, pos, avail);
         if (err) return err;
         if (act) stage_byte(sh, row, ent);
         if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;

@@ -38,7 +38,9 @@ constexpr u64 WS_PER_WAVE = WS_CODES + WS_FC;
 
 __device__ __forceinline__ u32 lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ u64 ballot(bool p) { return __ballot(p); }
+// The builtin on the bool itself: one v_cmp writing the lane mask (going
+// through an int predicate first costs a v_cndmask and a second v_cmp).
+__device__ __forceinline__ u64 ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // Number of set bits of `m` in lanes below this one (v_mbcnt_lo/hi).
 __device__ __forceinline__ u32 rank_below(u64 m)

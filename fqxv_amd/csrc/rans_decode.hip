@@ -44,7 +44,7 @@ constexpr u32 RING_MIRROR = 64;                      // words mirrored past the 
 constexpr u32 OUT_ROUNDS = 32;                       // rounds per output flush
 constexpr u32 OUT_BYTES = OUT_ROUNDS * NSTATES;      // 1 KiB
 constexpr u32 UMAX = 64;                             // order-1 fast path lane space
-constexpr u32 FAST_ROUNDS = 4;                       // rounds per branch-free block
+constexpr u32 O1_BLOCK = 8;                          // order-1 rounds per branch-free block
 // Order-1 code matrix rows: 64 codes plus 16 bytes of padding, so the
 // 16-byte row writes of consecutive lanes start on different LDS banks.
 constexpr u32 MROW = 80;
@@ -280,7 +280,7 @@ __device__ __forceinline__ int renorm(DecShared &sh, WordRing &wr, u32 &x, u32 v
 // (one ring chunk), so the ring's refill keeps up with a block at a time.
 constexpr u32 O0_BLOCK = 8;
 
-__device__ int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap)
+__device__ __attribute__((always_inline)) int decode_o0(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap)
 {
     const u32 l = lane_id();
     const u64 len = (u64)(end - s);
@@ -494,7 +494,7 @@ __device__ __forceinline__ u8 *table_base(u8 *ws)
 // those tables overflowed L2 (hit rate 40% against 97% at one per CU) and a
 // round slowed from 555 to 810 ns.  The distinct rows number 21 (DNA), about
 // 100 (names) and about 300 (K = 41 qualities) on the test corpora.
-template <bool LDS_ROWS>
+template <bool LDS_ROWS, u32 CW>
 __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K,
                                               u32 nu, u32 x, u64 hdr, u8 *ws, u32 rows_off)
 {
@@ -596,116 +596,164 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
     // Matrix bytes past the S copied per row stay zero (context 0).
 #pragma unroll
     for (u32 i = 0; i < 4; i++) ((uint4 *)mrow)[i] = make_uint4(0, 0, 0, 0);
-    // Each lane copies the code row of its own slot (lanes 32..63 hold
-    // x = 0 and fill rows nobody reads) into row `l` of the LDS matrix:
-    // mat[j][c] is the code context c decodes to at position j.  Wide
-    // per-lane loads and stores, no cross-lane broadcast of addresses.  A
-    // round's rows are loaded as soon as its states are known, at the end of
-    // the round before, so their latency overlaps that round's bookkeeping.
+    // Lane l < 32 copies the code row of its own slot into row `l` of the
+    // LDS matrix: mat[j][c] is the code context c decodes to at position j.
+    // Wide per-lane loads and stores, no cross-lane broadcast of addresses.
+    // A round's rows are loaded as soon as its states are known, at the end
+    // of the round before, so their latency overlaps that round's
+    // bookkeeping.  Rows of at most CW codes: CW / 16 chunks of 16 bytes.
+    constexpr u32 NCH = CW >= 16 ? CW / 16 : 1;
+    constexpr u32 REP = 64 / CW;          // lanes per context: groups split REP ways
+    constexpr u32 GPL = (NSTATES / 4) / REP; // groups of four positions per lane
     typedef u32 v4u __attribute__((ext_vector_type(4)));
-    v4u q0, q1, q2, q3;
-#define LOAD_ROWS(xs)                                                                                  \
-    do {                                                                                               \
-        const v4u *src_ = (const v4u *)(t8 + (u32)sh.rowidx[(xs) & (TOTFREQ - 1)] * S);               \
-        q0 = src_[0];                                                                                  \
-        q1 = src_[1];                                                                                  \
-        q2 = src_[2];                                                                                  \
-        q3 = src_[3];                                                                                  \
-    } while (0)
-    LOAD_ROWS(x);
+    v4u q[NCH];
+    // The row address is formed from the LDS array or the workspace pointer
+    // directly (through a generic pointer the loads become flat loads).
+    u8 *const tws = table_base(ws);
+    auto load_rows = [&](u32 ri) {
+        if (lo) {
+            const u32 off = __umul24(ri, S);
+            if constexpr (LDS_ROWS) {
+                const v4u *src_ = (const v4u *)((const u8 *)sh.tab + rows_off + off);
+#pragma unroll
+                for (u32 i = 0; i < NCH; i++) q[i] = src_[i];
+            } else {
+                const v4u *src_ = (const v4u *)(tws + off);
+#pragma unroll
+                for (u32 i = 0; i < NCH; i++) q[i] = src_[i];
+            }
+        }
+    };
+    auto store_rows = [&]() {
+        if (lo) {
+#pragma unroll
+            for (u32 i = 0; i < NCH; i++) ((v4u *)mrow)[i] = q[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+    };
+    // Lane L works for code context c = L % CW on the groups g = L / CW,
+    // L / CW + REP, ... (REP copies of the context lanes when the code space
+    // is small, so a round's matrix walk takes 8 / REP reads per level).
+    const u32 lc = REP > 1 ? l % CW : l;
+    const u32 mbase = REP > 1 ? MROW * 4 * (l / CW) : 0;
+    const u16 *ring16 = (const u16 *)sh.ring;
+    u32 ri = sh.rowidx[x & (TOTFREQ - 1)];
+    load_rows(ri);
     while (r < rounds) {
-        if (r + FAST_ROUNDS <= full && (row % FAST_ROUNDS) == 0) {
-            // FAST_ROUNDS rounds with no branch inside.  A round that meets an
+        u32 nblk = (u32)(((full - r) / O1_BLOCK) < 0x7fffffffull ? (full - r) / O1_BLOCK : 0x7fffffffull);
+        if ((row % O1_BLOCK) == 0 && nblk) {
+            // Branch-free blocks of O1_BLOCK rounds.  A round that meets an
             // absent context, a state needing a second word or the end of
             // the word stream marks its lanes bad; the whole block is then
             // undone and redone by the exact path one round at a time.  The
-            // ring holds every word the block can reach (at most 128 words,
-            // the ring keeps 512 ahead of the chunk start); words read past
-            // the stream end are never kept (the block is undone).
-            const u32 x0 = x, prev0 = prev;
-            const u64 pos0 = pos;
-            u32 bad = 0;
+            // ring holds every word a block can reach (at most 256 words
+            // consumed and 64 candidates past the last round's start; the
+            // ring keeps 512 ahead of the chunk start); words read past the
+            // stream end are never kept (the block is undone).
+            //
+            // Candidates, as in order-0: lane i holds the word at cb + i and
+            // the code row of the slot it would give (rowidx[w & 0xfff]); a
+            // round's renormalising lanes take both with one ds_bpermute at
+            // (pos - cb) + rank, the others look up rowidx[v & 0xfff] while
+            // the ballot resolves.  Lanes 32..63 carry meaningless states
+            // (indices stay in range, counts use the low 32 ballot bits,
+            // their output bytes go to `dense`, unused once the tables are
+            // built).
+            u64 cb = pos;
+            u32 cw = ring16[(u32)(pos % RING_WORDS) + l];
+            u32 cri = sh.rowidx[cw & (TOTFREQ - 1)];
+            for (; nblk; nblk--) {
+                const u32 x0 = x, prev0 = prev;
+                const u64 pos0 = pos;
+                u32 bad = 0;
 #pragma unroll
-            for (u32 t = 0; t < FAST_ROUNDS; t++) {
-                const u32 slot = x & (TOTFREQ - 1);
-                ((v4u *)mrow)[0] = q0;
-                ((v4u *)mrow)[1] = q1;
-                ((v4u *)mrow)[2] = q2;
-                ((v4u *)mrow)[3] = q3;
-                __builtin_amdgcn_wave_barrier();
-                asm volatile("" ::: "memory");
-                // Lane c walks four positions from context c at position 4g
-                // (three dependent LDS reads per group, all groups at once),
-                // so the chain takes one v_readlane per group: dg holds the
-                // four codes with the last one in byte 0, where the next
-                // step's lane select reads it.
-                u32 dg[NSTATES / 4], e1[NSTATES / 4], e2[NSTATES / 4], e0[NSTATES / 4], e3[NSTATES / 4];
+                for (u32 t = 0; t < O1_BLOCK; t++) {
+                    u32 nw = ring16[(u32)(pos % RING_WORDS) + l];
+                    asm volatile("" : "+v"(x)::"memory");
+                    const u32 slot = x & (TOTFREQ - 1);
+                    store_rows();
+                    // Lane (c, h) walks four positions from context c at
+                    // position 4g for each of its groups g (three dependent
+                    // LDS reads per group, all groups at once), so the
+                    // chain takes one v_readlane per group: dg holds the
+                    // four codes with the last one in byte 0, where the
+                    // next step's lane select reads it.
+                    u32 dg[GPL], e0[GPL], e1[GPL], e2[GPL], e3[GPL];
 #pragma unroll
-                for (u32 g = 0; g < NSTATES / 4; g++) e0[g] = sh.mat[MROW * (4 * g) + l];
+                    for (u32 i = 0; i < GPL; i++) e0[i] = sh.mat[mbase + MROW * (4 * REP * i) + lc];
 #pragma unroll
-                for (u32 g = 0; g < NSTATES / 4; g++) e1[g] = sh.mat[MROW * (4 * g + 1) + e0[g]];
+                    for (u32 i = 0; i < GPL; i++) e1[i] = sh.mat[mbase + MROW * (4 * REP * i + 1) + e0[i]];
 #pragma unroll
-                for (u32 g = 0; g < NSTATES / 4; g++) e2[g] = sh.mat[MROW * (4 * g + 2) + e1[g]];
+                    for (u32 i = 0; i < GPL; i++) e2[i] = sh.mat[mbase + MROW * (4 * REP * i + 2) + e1[i]];
 #pragma unroll
-                for (u32 g = 0; g < NSTATES / 4; g++) e3[g] = sh.mat[MROW * (4 * g + 3) + e2[g]];
-                // All eight last reads in flight before the first is used (the
-                // scheduler otherwise waits on each in turn in some kernels).
-                asm volatile("" : "+v"(e3[0]), "+v"(e3[1]), "+v"(e3[2]), "+v"(e3[3]), "+v"(e3[4]), "+v"(e3[5]),
-                             "+v"(e3[6]), "+v"(e3[7]));
+                    for (u32 i = 0; i < GPL; i++) e3[i] = sh.mat[mbase + MROW * (4 * REP * i + 3) + e2[i]];
+                    // All last reads in flight before the first is used.
 #pragma unroll
-                for (u32 g = 0; g < NSTATES / 4; g++) dg[g] = e3[g] | (e0[g] << 8) | (e1[g] << 16) | (e2[g] << 24);
-                u32 c = prev, dv = 0, cv = 0;
+                    for (u32 i = 0; i < GPL; i++) asm volatile("" : "+v"(e3[i]));
 #pragma unroll
-                for (u32 g = 0; g < NSTATES / 4; g++) {
-                    cv = writelane(c, 4 * g, cv);
-                    c = __builtin_amdgcn_readlane(dg[g], c);
-                    dv = writelane(c, 4 * g, dv);
+                    for (u32 i = 0; i < GPL; i++) dg[i] = e3[i] | (e0[i] << 8) | (e1[i] << 16) | (e2[i] << 24);
+                    u32 c = prev, dv = 0, cv = 0;
+#pragma unroll
+                    for (u32 g = 0; g < NSTATES / 4; g++) {
+                        cv = writelane(c, 4 * g, cv);
+                        // Byte 0 of c is below CW, so adding the lane group
+                        // never carries into the lane select's six bits.
+                        c = __builtin_amdgcn_readlane(dg[g / REP], uni(c + CW * (g % REP)));
+                        dv = writelane(c, 4 * g, dv);
+                    }
+                    prev = c & 0xffu;
+                    // Every lane of a group of four takes the group's dword;
+                    // lane 4g + k reads its symbol's code and its context's
+                    // code there.
+                    const u32 dq = (u32)__builtin_amdgcn_update_dpp(0, (int)dv, 0x00, 0xf, 0xf, false);
+                    const u32 cq = (u32)__builtin_amdgcn_update_dpp(0, (int)cv, 0x00, 0xf, 0xf, false);
+                    // Lane 4g takes its context from cv (the group's entry
+                    // context), lanes 4g+1..3 from dv: one byte select.
+                    const u32 k4 = l & 3;
+                    const u32 sc = (dq >> (8 * ((k4 + 1) & 3))) & 0xffu;
+                    const u32 cc = (((dq & 0xffffff00u) | (cq & 0xffu)) >> (8 * k4)) & 0xffu;
+                    const u32 ent = fc2[__umul24(cc & 63, nu) + (sc & 63)];
+                    const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
+                    const bool need = v < RANS_L;
+                    const u32 m = (u32)ballot(need);
+                    const u32 ia = ((u32)(pos - cb) + __builtin_amdgcn_mbcnt_lo(m, 0u)) << 2;
+                    u32 qn = (u32)__builtin_amdgcn_ds_bpermute((int)ia, (int)(cw | (cri << 16)));
+                    u32 riv = sh.rowidx[v & (TOTFREQ - 1)];
+                    asm volatile("" : "+v"(nw)::"memory");
+                    const u32 nri = sh.rowidx[nw & (TOTFREQ - 1)];
+                    asm volatile("" : "+v"(qn), "+v"(riv)::"memory");
+                    *(lo ? (u8 *)sh.outst + (row + t) * NSTATES + l : (u8 *)sh.dense + (l - NSTATES)) = (u8)ent;
+                    x = need ? ((v << 16) | (qn & 0xffffu)) : v;
+                    ri = need ? (qn >> 16) : riv;
+                    cb = pos;
+                    cw = nw;
+                    cri = nri;
+                    pos += (u32)__popc(m);
+                    bad |= (u32)ballot(x < RANS_L || cc >= K);
+                    load_rows(ri);
                 }
-                prev = c & 0xffu;
-                // Every lane of a group of four takes the group's dword; lane
-                // 4g + k reads its symbol's code and its context's code there.
-                const u32 dq = (u32)__builtin_amdgcn_update_dpp(0, (int)dv, 0x00, 0xf, 0xf, false);
-                const u32 cq = (u32)__builtin_amdgcn_update_dpp(0, (int)cv, 0x00, 0xf, 0xf, false);
-                // Lane 4g takes its context from cv (the group's entry
-                // context), lanes 4g+1..3 from dv: one byte select, no lane
-                // mask (a loop-invariant mask costs SGPRs the kernel spills).
-                const u32 k4 = l & 3;
-                const u32 sc = (dq >> (8 * ((k4 + 1) & 3))) & 0xffu;
-                const u32 cc = (((dq & 0xffffff00u) | (cq & 0xffu)) >> (8 * k4)) & 0xffu;
-                const u32 ent = fc2[(cc & 63) * nu + (sc & 63)];
-                const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
-                const bool need = lo && v < RANS_L;
-                const u64 m = ballot(need);
-                const u32 w = ((const u16 *)sh.ring)[(u32)(pos % RING_WORDS) + __builtin_amdgcn_mbcnt_lo((u32)m, 0u)];
-                if (lo) x = need ? ((v << 16) | w) : v;
-                pos += (u32)__popcll(m);
-                if (lo) stage_byte(sh, row + t, ent);
-                bad |= (u32)(lo && (x < RANS_L || cc >= K));
-                LOAD_ROWS(x);
-            }
-            if (ballot(bad != 0) == 0 && pos <= avail) {
-                r += FAST_ROUNDS;
-                row += FAST_ROUNDS;
+                if (bad || pos > avail) {
+                    x = x0;
+                    prev = prev0;
+                    pos = pos0;
+                    ri = sh.rowidx[x & (TOTFREQ - 1)];
+                    load_rows(ri);
+                    break;
+                }
+                r += O1_BLOCK;
+                row += O1_BLOCK;
                 wr.advance<LDS_ROWS>(sh.ring, pos);
                 if (row == OUT_ROUNDS) {
                     flush(sh, o, OUT_BYTES);
                     o += OUT_BYTES;
                     row = 0;
                 }
-                continue;
             }
-            x = x0;
-            prev = prev0;
-            pos = pos0;
-            LOAD_ROWS(x);
+            if (r >= rounds) break;
         }
         const u32 slot = x & (TOTFREQ - 1);
-        ((v4u *)mrow)[0] = q0;
-        ((v4u *)mrow)[1] = q1;
-        ((v4u *)mrow)[2] = q2;
-        ((v4u *)mrow)[3] = q3;
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("" ::: "memory");
+        store_rows();
         // Exact path: absent contexts, truncation, multi-word states, tail.
         if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT;
         const u32 cntr = r < full ? NSTATES : (u32)(n - r * NSTATES);
@@ -728,7 +776,8 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
         if (err) return err;
         if (act) stage_byte(sh, row, ent);
         if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
-        LOAD_ROWS(x);
+        ri = sh.rowidx[x & (TOTFREQ - 1)];
+        load_rows(ri);
         r++;
         if (++row == OUT_ROUNDS) {
             // The last round may be partial: never write past out + n.
@@ -740,7 +789,6 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
     }
     if (row) flush(sh, o, (u32)(n - (u64)(o - out)));
     return 0;
-#undef LOAD_ROWS
 }
 
 // Wide path: more than 64 contexts, or 64 contexts and one more symbol
@@ -868,7 +916,8 @@ __device__ int decode_o1_multi(DecShared &sh, const u8 *s, const u8 *end, u8 *ou
 }
 
 // rows_lds: the fast path may keep its code rows in LDS (job flags).
-__device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap, u8 *ws, bool rows_lds)
+__device__ __attribute__((always_inline)) int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u64 cap, u8 *ws,
+                                                     bool rows_lds)
 {
     const u32 l = lane_id();
     u32 K, symp;
@@ -903,16 +952,23 @@ __device__ int decode_o1(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64
         const u32 nrows = 1 + uni(wave_sum(__popc(bm[2 * l]) + __popc(bm[2 * l + 1])));
         const u32 S = (K + 15) & ~15u;
         const u32 rows_off = (K * nu * 4 + 15) & ~15u;
-        if (rows_lds && rows_off + nrows * S + 64 <= TAB_WORDS * 4)
-            return decode_o1_fast<true>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
-        return decode_o1_fast<false>(sh, s, end, out, n, K, nu, x, hdr, ws, 0);
+        // The code space (nu lane ids) in 8, 16, 32 or 64 lanes.
+        const bool in_lds = rows_lds && rows_off + nrows * S + 64 <= TAB_WORDS * 4;
+        if (in_lds) {
+            if (nu <= 8) return decode_o1_fast<true, 8>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
+            if (nu <= 16) return decode_o1_fast<true, 16>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
+            if (nu <= 32) return decode_o1_fast<true, 32>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
+            return decode_o1_fast<true, 64>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
+        }
+        if (nu <= 16) return decode_o1_fast<false, 16>(sh, s, end, out, n, K, nu, x, hdr, ws, 0);
+        return decode_o1_fast<false, 64>(sh, s, end, out, n, K, nu, x, hdr, ws, 0);
     }
     return decode_o1_multi(sh, s, end, out, n, K, x, hdr, ws);
 }
 
 // One job: header dispatch (scalar.rs:143-155, lib.rs:226-231).  Returns
 // the status; *n_out is the declared output length.
-__device__ int decode_job_core(DecShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, u8 *ws, u64 &n_out)
+__device__ __attribute__((always_inline)) int decode_job_core(DecShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, u8 *ws, u64 &n_out)
 {
     const u8 *s = src + jb.src_off;
     const u8 *end = s + jb.src_len;
@@ -947,7 +1003,7 @@ __device__ int decode_job_core(DecShared &sh, const u8 *src, u8 *dst, const fqxv
     return err;
 }
 
-__device__ void decode_job(DecShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, i32 *status,
+__device__ __attribute__((always_inline)) void decode_job(DecShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb, i32 *status,
                            u64 *out_len, u8 *ws)
 {
     u64 n = 0;

@@ -472,6 +472,39 @@ __device__ __forceinline__ u8 *table_base(u8 *ws)
     return ws + ((((u64)blockIdx.x * 2654435761u) & 0xffffffffu) >> 19 << 7);
 }
 
+// Distinct code rows.  The row of slot k (the code every context decodes at
+// k) changes only at the boundary slots o1_validate marked in sh.mat, so the
+// rows are numbered in slot order: rowidx[k] = number of boundaries in
+// [1, k].  There are at most 1 + the sum over contexts of their symbol counts
+// (tens to a few hundred), against 4096 slots.  Returns the row count.
+__device__ u32 build_rowidx(DecShared &sh)
+{
+    const u32 l = lane_id();
+    const u32 *bm = (const u32 *)sh.mat;
+    const u32 w0 = bm[2 * l], w1 = bm[2 * l + 1];
+    const u32 cnt = __popc(w0) + __popc(w1);
+    u32 base = wave_scan_incl(cnt) - cnt;
+#pragma unroll
+    for (u32 h = 0; h < 2; h++) {
+        const u32 w = h ? w1 : w0;
+#pragma unroll
+        for (u32 q = 0; q < 4; q++) {
+            u32 v[4];
+#pragma unroll
+            for (u32 i = 0; i < 4; i++) {
+                const u32 b = 8 * q + 2 * i; // slots 64l + 32h + b, b + 1
+                const u32 lo16 = base + __popc(w & ((2u << b) - 1));
+                const u32 hi16 = base + __popc(w & ((2u << (b + 1)) - 1));
+                v[i] = lo16 | (hi16 << 16);
+            }
+            *(uint4 *)&sh.rowidx[64 * l + 32 * h + 8 * q] = make_uint4(v[0], v[1], v[2], v[3]);
+        }
+        base += __popc(w);
+    }
+    __syncthreads();
+    return uni(sh.rowidx[TOTFREQ - 1]) + 1;
+}
+
 // Fast path: K <= 64 contexts and |contexts u symbols| = nu <= 64.
 // t8[row * S + d] (u8) = lane(sym) for each distinct code row, S = K rounded
 // up to 16, the bytes past K zero (context 0, a valid lane): in LDS behind
@@ -503,36 +536,9 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
     // wave's workspace.
     u8 *t8 = LDS_ROWS ? (u8 *)sh.tab + rows_off : table_base(ws);
     u32 *fc2 = sh.tab;
-    // Distinct code rows.  The row of slot k (the code every context decodes
-    // at k) changes only at the boundary slots o1_validate marked, so the
-    // rows are numbered in slot order, rowidx[k] = number of boundaries in
-    // [1, k], and t8 holds one row per number: at most 1 + sum over contexts
-    // of their symbol counts (tens to a few hundred), against 4096 per slot.
+    // Distinct code rows (see build_rowidx): t8 holds one row per number.
     // The small table stays in L1/L2 while many blocks decode at once.
-    {
-        const u32 *bm = (const u32 *)sh.mat;
-        const u32 w0 = bm[2 * l], w1 = bm[2 * l + 1];
-        const u32 cnt = __popc(w0) + __popc(w1);
-        u32 base = wave_scan_incl(cnt) - cnt;
-#pragma unroll
-        for (u32 h = 0; h < 2; h++) {
-            const u32 w = h ? w1 : w0;
-#pragma unroll
-            for (u32 q = 0; q < 4; q++) {
-                u32 v[4];
-#pragma unroll
-                for (u32 i = 0; i < 4; i++) {
-                    const u32 b = 8 * q + 2 * i; // slots 64l + 32h + b, b + 1
-                    const u32 lo16 = base + __popc(w & ((2u << b) - 1));
-                    const u32 hi16 = base + __popc(w & ((2u << (b + 1)) - 1));
-                    v[i] = lo16 | (hi16 << 16);
-                }
-                *(uint4 *)&sh.rowidx[64 * l + 32 * h + 8 * q] = make_uint4(v[0], v[1], v[2], v[3]);
-            }
-            base += __popc(w);
-        }
-    }
-    const u32 nrows = uni(sh.rowidx[TOTFREQ - 1]) + 1;
+    const u32 nrows = build_rowidx(sh);
     // Rows are 16-byte aligned: S >= K bytes each.  A round reads 64 bytes
     // from each row start: past S they are the codes of the next rows, and
     // the last row's over-read stays inside the region (the caller leaves
@@ -791,6 +797,256 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
     return 0;
 }
 
+// Register path: a code space of at most eight lane ids (nu <= 8: DNA-like
+// blocks).  The map "context -> code of the symbol decoded" of one slot is
+// eight 3-bit codes, held per lane as two byte maps (contexts 0..3 and
+// 4..7), so the composition of two maps is two v_perm_b32 (each selector
+// byte picks a byte of the 8-byte table).  Per round, lane j takes the map of
+// its own state's slot and an inclusive scan over lanes 0..31 (row_shr 1, 2,
+// 4, 8 within 16-lane rows, then row_bcast 15 into the second row: five DPP
+// steps) leaves lane j the composition of maps 0..j; applied to the round's
+// incoming context it is position j's symbol, and a one-lane wave shift
+// gives each position its context.  The state entry comes from the row of
+// entries of the slot's code row (eight contexts, loaded while the scan runs)
+// by the context's three bits, so the chain has no LDS access between the
+// round's state and the next one except the renormalisation word.
+//
+// LDS (sh.tab, words): MT[slot] the packed map of each slot (bits 8i..8i+3
+// the code of context i, bits 8i+4..8i+7 that of context 4 + i), ER[row * 8
+// + c] the entry (f-1)<<20 | cum<<8 | sym of context c at each distinct code
+// row, PRC[row] the packed map of each row.  Codes of contexts >= K are 0
+// and their entries 0: a position that meets one is flagged and redone by
+// the exact path, which reports the absent context.
+constexpr u32 PERM_MT = 0;
+constexpr u32 PERM_ER = TOTFREQ;
+constexpr u32 PERM_ROWS_MAX = 1 + 8 * 8;  // 1 + sum over <= 8 contexts of <= 8 symbols
+constexpr u32 PERM_PRC = PERM_ER + 8 * PERM_ROWS_MAX;
+static_assert(PERM_PRC + PERM_ROWS_MAX <= TAB_WORDS, "register-path tables fit the decode table area");
+
+// One scan step: lane j's map becomes (its map) o (the map of the lane CTRL
+// names), applied right to left; lanes without a source lane keep theirs.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void compose_step(u32 &lo, u32 &hi)
+{
+    const u32 tl = (u32)__builtin_amdgcn_update_dpp((int)0x03020100, (int)lo, CTRL, RMASK, 0xf, false);
+    const u32 th = (u32)__builtin_amdgcn_update_dpp((int)0x07060504, (int)hi, CTRL, RMASK, 0xf, false);
+    const u32 nl = __builtin_amdgcn_perm(hi, lo, tl);
+    const u32 nh = __builtin_amdgcn_perm(hi, lo, th);
+    lo = nl;
+    hi = nh;
+}
+
+// Packed map w of position j (lanes 0..31) -> code of the symbol at j and
+// code of its context, given the round's incoming context code cin (< 8).
+__device__ __forceinline__ void perm_round_codes(u32 w, u32 cin, u32 &sc, u32 &cc)
+{
+    u32 lo = w & 0x0f0f0f0fu, hi = (w >> 4) & 0x0f0f0f0fu;
+    compose_step<0x111, 0xf>(lo, hi); // row_shr:1
+    compose_step<0x112, 0xf>(lo, hi); // row_shr:2
+    compose_step<0x114, 0xf>(lo, hi); // row_shr:4
+    compose_step<0x118, 0xf>(lo, hi); // row_shr:8
+    compose_step<0x142, 0xa>(lo, hi); // row_bcast:15 into rows 1 and 3
+    // Selector byte 0 = cin, bytes 1..3 = 12 (a zero byte).
+    sc = __builtin_amdgcn_perm(hi, lo, 0x0c0c0c00u | cin);
+    cc = (u32)__builtin_amdgcn_update_dpp((int)cin, (int)sc, 0x138, 0xf, 0xf, false); // wave_shr:1
+}
+
+__device__ __forceinline__ u32 perm_entry(uint4 ea, uint4 eb, u32 cc)
+{
+    const bool b0 = cc & 1;
+    const u32 s01 = b0 ? ea.y : ea.x, s23 = b0 ? ea.w : ea.z;
+    const u32 s45 = b0 ? eb.y : eb.x, s67 = b0 ? eb.w : eb.z;
+    const u32 s03 = (cc & 2) ? s23 : s01, s47 = (cc & 2) ? s67 : s45;
+    return (cc & 4) ? s47 : s03;
+}
+
+__device__ __forceinline__ int decode_o1_perm(DecShared &sh, const u8 *s, const u8 *end, u8 *out, u64 n, u32 K,
+                                              u32 x, u64 hdr)
+{
+    const u32 l = lane_id();
+    u32 *const MT = sh.tab + PERM_MT;
+    u32 *const ER = sh.tab + PERM_ER;
+    u32 *const PRC = sh.tab + PERM_PRC;
+    const u32 nrows = build_rowidx(sh);
+    for (u32 i = l; i < 8 * nrows; i += 64) ER[i] = 0;
+    for (u32 i = l; i < nrows; i += 64) PRC[i] = 0;
+    __syncthreads();
+    {
+        // Each row starts at one slot (slot 0 or a boundary), owned by one
+        // lane, which writes that row's codes and entries context by context.
+        const u32 *bm = (const u32 *)sh.mat;
+        const u64 starts = ((u64)bm[2 * l + 1] << 32 | bm[2 * l]) | (l == 0 ? 1ull : 0ull);
+        const u32 row0 = l == 0 ? 0u : (u32)sh.rowidx[64 * l] - (u32)(bm[2 * l] & 1u) + 1u;
+        const u8 *t = s + HDR0 + BITMAP;
+        for (u32 d = 0; d < K; d++, t += 512) {
+            (void)load_freq_table(sh, t, end); // validated already
+            if (starts) {
+                // Owner of slot 64l: the largest symbol with cum <= 64l.
+                const u32 k0 = 64 * l;
+                u32 lo = 0, hi = 255;
+                while (lo < hi) {
+                    const u32 mid = (lo + hi + 1) / 2;
+                    if ((sh.fc[mid] & 0xffff) <= k0)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                u32 sy = lo, fcs = sh.fc[sy], ri = row0;
+                const u32 sh4 = d < 4 ? 8 * d : 8 * (d - 4) + 4;
+                for (u64 rs = starts; rs; rs &= rs - 1, ri++) {
+                    const u32 k = k0 + (u32)__builtin_ctzll(rs);
+                    while ((fcs & 0xffff) + (fcs >> 16) <= k) fcs = sh.fc[++sy];
+                    PRC[ri] |= (u32)sh.ulane[sy] << sh4;
+                    ER[8 * ri + d] = (((fcs >> 16) - 1) << 20) | ((fcs & 0xffff) << 8) | sy;
+                }
+            }
+            __syncthreads();
+        }
+        // MT[k] = PRC[rowidx[k]], lane l for slots [64l, 64l + 64).
+#pragma unroll 2
+        for (u32 k = 64 * l; k < 64 * l + 64; k += 8) {
+            const uint4 r4 = *(const uint4 *)&sh.rowidx[k];
+            const u32 rr[4] = {r4.x, r4.y, r4.z, r4.w};
+            u32 v[8];
+#pragma unroll
+            for (u32 i = 0; i < 4; i++) {
+                v[2 * i] = PRC[rr[i] & 0xffff];
+                v[2 * i + 1] = PRC[rr[i] >> 16];
+            }
+            *(uint4 *)&MT[k] = make_uint4(v[0], v[1], v[2], v[3]);
+            *(uint4 *)&MT[k + 4] = make_uint4(v[4], v[5], v[6], v[7]);
+        }
+        __syncthreads();
+    }
+
+    WordRing wr;
+    wr.init(s + hdr, end);
+    const u64 avail = ((u64)(end - s) - hdr) / 2;
+    wr.reset<true>(sh.ring, 0);
+
+    u32 prev = sh.dense[0]; // context of symbol 0 is 0 (scalar.rs:203); >= K if absent
+    const bool lo = l < NSTATES;
+    const u64 rounds = (n + NSTATES - 1) / NSTATES;
+    const u64 full = n / NSTATES;
+    u64 pos = 0;
+    u32 row = 0;
+    u8 *o = out;
+    u64 r = 0;
+    const u16 *ring16 = (const u16 *)sh.ring;
+    // This lane's slot: its packed map and its code row's entries.
+    u32 mt = MT[x & (TOTFREQ - 1)];
+    u32 ri = sh.rowidx[x & (TOTFREQ - 1)];
+    uint4 ea = *(const uint4 *)&ER[8 * ri], eb = *(const uint4 *)&ER[8 * ri + 4];
+    while (r < rounds) {
+        u32 nblk = (u32)(((full - r) / O1_BLOCK) < 0x7fffffffull ? (full - r) / O1_BLOCK : 0x7fffffffull);
+        if ((row % O1_BLOCK) == 0 && nblk && prev < K) {
+            // Branch-free blocks of O1_BLOCK rounds, undone and redone by the
+            // exact path when a round meets an absent context, a state that
+            // needs a second word, or the end of the words (as in
+            // decode_o1_fast).  Candidates: lane i holds the word at cb + i,
+            // the packed map and the row of the slot it would give; a round's
+            // renormalising lanes take them with ds_bpermute at
+            // (pos - cb) + rank, the others look up v & 0xfff while the ballot
+            // resolves.  Lanes 32..63 carry meaningless states (indices stay
+            // in range, counts use the low 32 ballot bits, their output
+            // bytes go to `dense`, unused once the tables are built).
+            u64 cb = pos;
+            u32 cw = ring16[(u32)(pos % RING_WORDS) + l];
+            u32 cmt = MT[cw & (TOTFREQ - 1)];
+            u32 cri = sh.rowidx[cw & (TOTFREQ - 1)];
+            for (; nblk; nblk--) {
+                const u32 x0 = x, prev0 = prev;
+                const u64 pos0 = pos;
+                u32 bad = 0;
+#pragma unroll
+                for (u32 t = 0; t < O1_BLOCK; t++) {
+                    u32 nw = ring16[(u32)(pos % RING_WORDS) + l];
+                    asm volatile("" : "+v"(x), "+v"(mt)::"memory");
+                    const u32 slot = x & (TOTFREQ - 1);
+                    u32 sc, cc;
+                    perm_round_codes(mt, prev, sc, cc);
+                    const u32 ent = perm_entry(ea, eb, cc);
+                    const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
+                    const bool need = v < RANS_L;
+                    const u32 m = (u32)ballot(need);
+                    const u32 ia = ((u32)(pos - cb) + __builtin_amdgcn_mbcnt_lo(m, 0u)) << 2;
+                    u32 qn = (u32)__builtin_amdgcn_ds_bpermute((int)ia, (int)(cw | (cri << 16)));
+                    u32 qm = (u32)__builtin_amdgcn_ds_bpermute((int)ia, (int)cmt);
+                    u32 mv = MT[v & (TOTFREQ - 1)];
+                    u32 rv = sh.rowidx[v & (TOTFREQ - 1)];
+                    asm volatile("" : "+v"(nw)::"memory");
+                    const u32 nmt = MT[nw & (TOTFREQ - 1)];
+                    const u32 nri = sh.rowidx[nw & (TOTFREQ - 1)];
+                    asm volatile("" : "+v"(qn), "+v"(qm), "+v"(mv), "+v"(rv)::"memory");
+                    *(lo ? (u8 *)sh.outst + (row + t) * NSTATES + l : (u8 *)sh.dense + (l - NSTATES)) = (u8)ent;
+                    x = need ? ((v << 16) | (qn & 0xffffu)) : v;
+                    ri = need ? (qn >> 16) : rv;
+                    mt = need ? qm : mv;
+                    ea = *(const uint4 *)&ER[8 * ri];
+                    eb = *(const uint4 *)&ER[8 * ri + 4];
+                    cb = pos;
+                    cw = nw;
+                    cmt = nmt;
+                    cri = nri;
+                    pos += (u32)__popc(m);
+                    bad |= (u32)ballot(x < RANS_L || cc >= K);
+                    prev = __builtin_amdgcn_readlane(sc, NSTATES - 1);
+                }
+                if (bad || pos > avail) {
+                    x = x0;
+                    prev = prev0;
+                    pos = pos0;
+                    mt = MT[x & (TOTFREQ - 1)];
+                    ri = sh.rowidx[x & (TOTFREQ - 1)];
+                    ea = *(const uint4 *)&ER[8 * ri];
+                    eb = *(const uint4 *)&ER[8 * ri + 4];
+                    break;
+                }
+                r += O1_BLOCK;
+                row += O1_BLOCK;
+                wr.advance<true>(sh.ring, pos);
+                if (row == OUT_ROUNDS) {
+                    flush(sh, o, OUT_BYTES);
+                    o += OUT_BYTES;
+                    row = 0;
+                }
+            }
+            if (r >= rounds) break;
+        }
+        // Exact path: absent contexts, truncation, multi-word states, tail.
+        if (prev >= K) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        const u32 slot = x & (TOTFREQ - 1);
+        const u32 cntr = r < full ? NSTATES : (u32)(n - r * NSTATES);
+        u32 sc, cc;
+        perm_round_codes(mt, prev, sc, cc);
+        // First position whose context is no context: that symbol fails.
+        const u64 nb = ballot(lo && l < cntr && cc >= K);
+        const u32 jb = nb ? (u32)__builtin_ctzll(nb) : NSTATES;
+        const bool act = lo && l < cntr && l < jb;
+        const u32 ent = perm_entry(ea, eb, cc);
+        const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
+        int err = renorm<true>(sh, wr, x, v, act, pos, avail);
+        if (err) return err;
+        if (act) stage_byte(sh, row, ent);
+        if (jb < cntr) return FQXV_RANS_ERR_ABSENT_CONTEXT;
+        prev = __builtin_amdgcn_readlane(sc, NSTATES - 1);
+        mt = MT[x & (TOTFREQ - 1)];
+        ri = sh.rowidx[x & (TOTFREQ - 1)];
+        ea = *(const uint4 *)&ER[8 * ri];
+        eb = *(const uint4 *)&ER[8 * ri + 4];
+        r++;
+        if (++row == OUT_ROUNDS) {
+            // The last round may be partial: never write past out + n.
+            const u64 left = n - (u64)(o - out);
+            flush(sh, o, left < OUT_BYTES ? (u32)left : OUT_BYTES);
+            o += OUT_BYTES;
+            row = 0;
+        }
+    }
+    if (row) flush(sh, o, (u32)(n - (u64)(o - out)));
+    return 0;
+}
+
 // Wide path: more than 64 contexts, or 64 contexts and one more symbol
 // (|contexts u symbols| <= 256).  The same column gather and v_readlane chain
 // as the fast path, with four contexts packed per lane: byte q of lane l of
@@ -944,6 +1200,7 @@ __device__ __attribute__((always_inline)) int decode_o1(DecShared &sh, const u8 
         }
     }
     __syncthreads();
+    if (nu <= 8) return decode_o1_perm(sh, s, end, out, n, K, x, hdr);
     if (K <= UMAX && nu <= UMAX) {
         // Distinct code rows (1 + the boundary slots o1_validate marked) and
         // where they go: LDS behind fc2[K][nu] when they fit with the 64-byte
@@ -952,10 +1209,9 @@ __device__ __attribute__((always_inline)) int decode_o1(DecShared &sh, const u8 
         const u32 nrows = 1 + uni(wave_sum(__popc(bm[2 * l]) + __popc(bm[2 * l + 1])));
         const u32 S = (K + 15) & ~15u;
         const u32 rows_off = (K * nu * 4 + 15) & ~15u;
-        // The code space (nu lane ids) in 8, 16, 32 or 64 lanes.
+        // The code space (nu lane ids) in 16, 32 or 64 lanes.
         const bool in_lds = rows_lds && rows_off + nrows * S + 64 <= TAB_WORDS * 4;
         if (in_lds) {
-            if (nu <= 8) return decode_o1_fast<true, 8>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
             if (nu <= 16) return decode_o1_fast<true, 16>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
             if (nu <= 32) return decode_o1_fast<true, 32>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);
             return decode_o1_fast<true, 64>(sh, s, end, out, n, K, nu, x, hdr, ws, rows_off);

@@ -823,13 +823,28 @@ constexpr u32 PERM_ROWS_MAX = 1 + 8 * 8;  // 1 + sum over <= 8 contexts of <= 8 
 constexpr u32 PERM_PRC = PERM_ER + 8 * PERM_ROWS_MAX;
 static_assert(PERM_PRC + PERM_ROWS_MAX <= TAB_WORDS, "register-path tables fit the decode table area");
 
+// DPP temporaries of the scan, one pair per step.  A step's lanes without a
+// source lane are the same in every round (lanes l % 16 < d for row_shr:d,
+// rows 0 and 2 for the row broadcast), and DPP leaves them unwritten, so the
+// identity map put there once before the first round stays (no per-round
+// re-initialisation of the DPP destinations).
+struct ScanTemps {
+    u32 l1, h1, l2, h2, l4, h4, l8, h8, lb, hb;
+    __device__ void init()
+    {
+        l1 = l2 = l4 = l8 = lb = 0x03020100u;
+        h1 = h2 = h4 = h8 = hb = 0x07060504u;
+    }
+};
+
 // One scan step: lane j's map becomes (its map) o (the map of the lane CTRL
-// names), applied right to left; lanes without a source lane keep theirs.
+// names), applied right to left; lanes without a source lane keep theirs
+// (their temporaries hold the identity).
 template <int CTRL, int RMASK>
-__device__ __forceinline__ void compose_step(u32 &lo, u32 &hi)
+__device__ __forceinline__ void compose_step(u32 &lo, u32 &hi, u32 &tl, u32 &th)
 {
-    const u32 tl = (u32)__builtin_amdgcn_update_dpp((int)0x03020100, (int)lo, CTRL, RMASK, 0xf, false);
-    const u32 th = (u32)__builtin_amdgcn_update_dpp((int)0x07060504, (int)hi, CTRL, RMASK, 0xf, false);
+    tl = (u32)__builtin_amdgcn_update_dpp((int)tl, (int)lo, CTRL, RMASK, 0xf, false);
+    th = (u32)__builtin_amdgcn_update_dpp((int)th, (int)hi, CTRL, RMASK, 0xf, false);
     const u32 nl = __builtin_amdgcn_perm(hi, lo, tl);
     const u32 nh = __builtin_amdgcn_perm(hi, lo, th);
     lo = nl;
@@ -838,14 +853,14 @@ __device__ __forceinline__ void compose_step(u32 &lo, u32 &hi)
 
 // Packed map w of position j (lanes 0..31) -> code of the symbol at j and
 // code of its context, given the round's incoming context code cin (< 8).
-__device__ __forceinline__ void perm_round_codes(u32 w, u32 cin, u32 &sc, u32 &cc)
+__device__ __forceinline__ void perm_round_codes(u32 w, u32 cin, ScanTemps &tp, u32 &sc, u32 &cc)
 {
     u32 lo = w & 0x0f0f0f0fu, hi = (w >> 4) & 0x0f0f0f0fu;
-    compose_step<0x111, 0xf>(lo, hi); // row_shr:1
-    compose_step<0x112, 0xf>(lo, hi); // row_shr:2
-    compose_step<0x114, 0xf>(lo, hi); // row_shr:4
-    compose_step<0x118, 0xf>(lo, hi); // row_shr:8
-    compose_step<0x142, 0xa>(lo, hi); // row_bcast:15 into rows 1 and 3
+    compose_step<0x111, 0xf>(lo, hi, tp.l1, tp.h1); // row_shr:1
+    compose_step<0x112, 0xf>(lo, hi, tp.l2, tp.h2); // row_shr:2
+    compose_step<0x114, 0xf>(lo, hi, tp.l4, tp.h4); // row_shr:4
+    compose_step<0x118, 0xf>(lo, hi, tp.l8, tp.h8); // row_shr:8
+    compose_step<0x142, 0xa>(lo, hi, tp.lb, tp.hb); // row_bcast:15 into rows 1 and 3
     // Selector byte 0 = cin, bytes 1..3 = 12 (a zero byte).
     sc = __builtin_amdgcn_perm(hi, lo, 0x0c0c0c00u | cin);
     cc = (u32)__builtin_amdgcn_update_dpp((int)cin, (int)sc, 0x138, 0xf, 0xf, false); // wave_shr:1
@@ -937,6 +952,8 @@ __device__ __forceinline__ int decode_o1_perm(DecShared &sh, const u8 *s, const 
     u32 mt = MT[x & (TOTFREQ - 1)];
     u32 ri = sh.rowidx[x & (TOTFREQ - 1)];
     uint4 ea = *(const uint4 *)&ER[8 * ri], eb = *(const uint4 *)&ER[8 * ri + 4];
+    ScanTemps tp;
+    tp.init();
     while (r < rounds) {
         u32 nblk = (u32)(((full - r) / O1_BLOCK) < 0x7fffffffull ? (full - r) / O1_BLOCK : 0x7fffffffull);
         if ((row % O1_BLOCK) == 0 && nblk && prev < K) {
@@ -964,7 +981,7 @@ __device__ __forceinline__ int decode_o1_perm(DecShared &sh, const u8 *s, const 
                     asm volatile("" : "+v"(x), "+v"(mt)::"memory");
                     const u32 slot = x & (TOTFREQ - 1);
                     u32 sc, cc;
-                    perm_round_codes(mt, prev, sc, cc);
+                    perm_round_codes(mt, prev, tp, sc, cc);
                     const u32 ent = perm_entry(ea, eb, cc);
                     const u32 v = ((ent >> 20) + 1) * (x >> SCALE_BITS) + slot - ((ent >> 8) & 0xfff);
                     const bool need = v < RANS_L;
@@ -1018,7 +1035,7 @@ __device__ __forceinline__ int decode_o1_perm(DecShared &sh, const u8 *s, const 
         const u32 slot = x & (TOTFREQ - 1);
         const u32 cntr = r < full ? NSTATES : (u32)(n - r * NSTATES);
         u32 sc, cc;
-        perm_round_codes(mt, prev, sc, cc);
+        perm_round_codes(mt, prev, tp, sc, cc);
         // First position whose context is no context: that symbol fails.
         const u64 nb = ballot(lo && l < cntr && cc >= K);
         const u32 jb = nb ? (u32)__builtin_ctzll(nb) : NSTATES;

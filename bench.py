@@ -749,12 +749,14 @@ def cpu_model():
 def bench_cpu(args, oracle, sample, workload):
     host_c, host_off, lens, sizes, orders = sample
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    # The oracle compiled for the highest x86-64 level this host supports.
+    blib, bflags = oracle.baseline_lib()
 
     def run(idx, nthreads, min_s):
         nbytes, t, passes = 0, 0.0, 0
         while t < min_s:
             t0 = time.perf_counter()
-            _, _, ol, st = oracle.decode_many(host_c, host_off[idx], lens[idx], sizes[idx], nthreads)
+            _, _, ol, st = oracle.decode_many(host_c, host_off[idx], lens[idx], sizes[idx], nthreads, use=blib)
             t += time.perf_counter() - t0
             assert np.all(st == 0)
             nbytes += int(sizes[idx].sum())
@@ -773,7 +775,7 @@ def bench_cpu(args, oracle, sample, workload):
             "single_thread": {"value": round(gbs_1, 4), "unit": "GB/s",
                               "sample": f"every 10th block of that sample ({len(sub)} blocks), {p_1}x, "
                                         f"{t_1:.1f}s"},
-            "host_cpu": cpu_model(), "nproc": os.cpu_count()}
+            "build": f"oracle/rans_oracle.c, gcc {bflags}", "host_cpu": cpu_model(), "nproc": os.cpu_count()}
 
 
 def main():

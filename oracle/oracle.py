@@ -57,37 +57,82 @@ def build(force: bool = False) -> str:
 _lib = None
 
 
+def _host_level() -> int:
+    """Highest x86-64 micro-architecture level of the running CPU (2, 3 or 4)."""
+    try:
+        flags = set()
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        return 2
+    v3 = {"avx", "avx2", "bmi1", "bmi2", "f16c", "fma", "movbe", "xsave"}
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    if not v3 <= flags:
+        return 2
+    return 4 if v4 <= flags else 3
+
+
+def baseline_build() -> tuple[str, str]:
+    """The oracle build the CPU baseline times: the one compiled for the
+    highest level this host supports (path, compiler flags)."""
+    level = _host_level()
+    while level > 2:
+        path = os.path.join(HERE, "_build", f"librans_oracle_v{level}.so")
+        if os.path.exists(path):
+            return path, f"-O3 -march=x86-64-v{level}"
+        level -= 1
+    return LIB_PATH, "-O3 -march=x86-64-v2"
+
+
+def _declare(L):
+    c_sz = ctypes.c_size_t
+    p8 = ctypes.c_void_p
+    L.orc_encode_bound.restype = c_sz
+    L.orc_encode_bound.argtypes = [c_sz, ctypes.c_int]
+    L.orc_encode.argtypes = [p8, c_sz, ctypes.c_int, p8, c_sz, ctypes.POINTER(c_sz)]
+    L.orc_decode.argtypes = [p8, c_sz, p8, c_sz, ctypes.POINTER(c_sz), ctypes.POINTER(c_sz)]
+    L.orc_decode_bounded.argtypes = [p8, c_sz, c_sz, p8, c_sz, ctypes.POINTER(c_sz)]
+    L.orc_peek_len.argtypes = [p8, c_sz, ctypes.POINTER(ctypes.c_uint64)]
+    L.orc_normalize.argtypes = [p8, p8]
+    L.orc_validate_freqs.argtypes = [p8]
+    L.orc_encsym_apply.restype = ctypes.c_uint32
+    L.orc_encsym_apply.argtypes = [p8, ctypes.c_uint32]
+    L.orc_encsym_new.argtypes = [ctypes.c_uint32, ctypes.c_uint32, p8]
+    for fn in (L.orc_encode_many, L.orc_decode_many):
+        fn.restype = ctypes.c_int
+    L.orc_encode_many.argtypes = [p8, p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
+    L.orc_decode_many.argtypes = [p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
+    L.orc_crc32c.restype = ctypes.c_uint32
+    L.orc_crc32c.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    L.orc_crc32c_combine.restype = ctypes.c_uint32
+    L.orc_crc32c_combine.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+    L.orc_crc32c_many.restype = None
+    L.orc_crc32c_many.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32, ctypes.c_void_p]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        c_sz = ctypes.c_size_t
-        p8 = ctypes.c_void_p
-        L.orc_encode_bound.restype = c_sz
-        L.orc_encode_bound.argtypes = [c_sz, ctypes.c_int]
-        L.orc_encode.argtypes = [p8, c_sz, ctypes.c_int, p8, c_sz, ctypes.POINTER(c_sz)]
-        L.orc_decode.argtypes = [p8, c_sz, p8, c_sz, ctypes.POINTER(c_sz), ctypes.POINTER(c_sz)]
-        L.orc_decode_bounded.argtypes = [p8, c_sz, c_sz, p8, c_sz, ctypes.POINTER(c_sz)]
-        L.orc_peek_len.argtypes = [p8, c_sz, ctypes.POINTER(ctypes.c_uint64)]
-        L.orc_normalize.argtypes = [p8, p8]
-        L.orc_validate_freqs.argtypes = [p8]
-        L.orc_encsym_apply.restype = ctypes.c_uint32
-        L.orc_encsym_apply.argtypes = [p8, ctypes.c_uint32]
-        L.orc_encsym_new.argtypes = [ctypes.c_uint32, ctypes.c_uint32, p8]
-        for fn in (L.orc_encode_many, L.orc_decode_many):
-            fn.restype = ctypes.c_int
-        L.orc_encode_many.argtypes = [p8, p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
-        L.orc_decode_many.argtypes = [p8, p8, p8, c_sz, p8, p8, p8, p8, p8, ctypes.c_int]
-        L.orc_crc32c.restype = ctypes.c_uint32
-        L.orc_crc32c.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
-        L.orc_crc32c_combine.restype = ctypes.c_uint32
-        L.orc_crc32c_combine.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
-        L.orc_crc32c_many.restype = None
-        L.orc_crc32c_many.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32, ctypes.c_void_p]
-        _lib = L
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+_baseline = None
+
+
+def baseline_lib():
+    """The oracle built for this host's highest x86-64 level (CPU baseline)."""
+    global _baseline
+    if _baseline is None:
+        path, flags = baseline_build()
+        _baseline = (_declare(ctypes.CDLL(path)), flags)
+    return _baseline
 
 
 def _ptr(a: np.ndarray):
@@ -211,7 +256,7 @@ def encode_many(arena: np.ndarray, offs, lens, orders, threads: int):
     return dst, doff, out_len, status
 
 
-def decode_many(arena: np.ndarray, offs, lens, out_caps, threads: int):
+def decode_many(arena: np.ndarray, offs, lens, out_caps, threads: int, use=None):
     offs = np.ascontiguousarray(offs, dtype=np.uint64)
     lens = np.ascontiguousarray(lens, dtype=np.uint64)
     caps = np.ascontiguousarray(out_caps, dtype=np.uint64)
@@ -221,7 +266,7 @@ def decode_many(arena: np.ndarray, offs, lens, out_caps, threads: int):
     dst = np.empty(int(caps.sum()) + 1, dtype=np.uint8)
     out_len = np.zeros(len(lens), dtype=np.uint64)
     status = np.zeros(len(lens), dtype=np.int32)
-    lib().orc_decode_many(_ptr(arena), _ptr(offs), _ptr(lens), len(lens), _ptr(dst), _ptr(doff),
+    (use or lib()).orc_decode_many(_ptr(arena), _ptr(offs), _ptr(lens), len(lens), _ptr(dst), _ptr(doff),
                           _ptr(caps), _ptr(out_len), _ptr(status), threads)
     return dst, doff, out_len, status
 

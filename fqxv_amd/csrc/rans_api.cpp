@@ -29,7 +29,8 @@ hipError_t fqxv_launch_decode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *
 hipError_t fqxv_decode_occupancy(int *);
 size_t fqxv_ws_per_wave();
 hipError_t fqxv_launch_encode_batch(const u8 *, u8 *, const fqxv_rans_job *, u32, const u32 *, u32 *, i32 *,
-                                    u64 *, u8 *, u32, hipStream_t, int);
+                                    u64 *, u8 *, u32, hipStream_t, int, void *);
+size_t fqxv_enc_pre_bytes();
 hipError_t fqxv_launch_encode_one(const u8 *, u8 *, const fqxv_rans_job &, i32 *, u64 *, u8 *, hipStream_t);
 hipError_t fqxv_encode_occupancy(int *);
 hipError_t fqxv_launch_perm_split(const u32 *, u64, u8 *, hipStream_t);
@@ -87,6 +88,7 @@ struct fqxv_rans_ctx {
     u64 xxh_cap = 0;
     u8 *cat_buf = nullptr;                  // concatenated-digest staging and copy items
     u64 cat_cap = 0;
+    void *pre = nullptr;                    // encode histogram pre-pass (fixed size, never moves)
 };
 
 namespace {
@@ -118,6 +120,18 @@ int ensure_ws(fqxv_rans_ctx *c, u64 waves)
         return FQXV_RANS_ERR_DEVICE;
     }
     c->ws_waves = waves;
+    return FQXV_RANS_OK;
+}
+
+// The encode pre-pass buffer: allocated once per context at its fixed size,
+// so a captured graph's address of it stays valid.
+int ensure_pre(fqxv_rans_ctx *c)
+{
+    if (c->pre) return FQXV_RANS_OK;
+    if (hipMalloc(&c->pre, fqxv_enc_pre_bytes()) != hipSuccess) {
+        c->pre = nullptr;
+        return FQXV_RANS_ERR_DEVICE;
+    }
     return FQXV_RANS_OK;
 }
 
@@ -249,6 +263,7 @@ void fqxv_rans_ctx_destroy(fqxv_rans_ctx *c)
         if (c->stream) (void)hipStreamSynchronize(c->stream);
         (void)hipDeviceSynchronize();
         if (c->ws) (void)hipFree(c->ws);
+        if (c->pre) (void)hipFree(c->pre);
         if (c->counters) (void)hipFree(c->counters);
         if (c->d_in) (void)hipFree(c->d_in);
         if (c->d_out) (void)hipFree(c->d_out);
@@ -397,11 +412,12 @@ static int batch_common(fqxv_rans_ctx *c, const uint8_t *d_src, uint8_t *d_dst, 
     DeviceGuard g(c->device);
     u32 grid = std::min<u32>(count, enc ? c->enc_waves : c->dec_waves);
     int rc = ensure_ws(c, grid);
+    if (!rc && enc) rc = ensure_pre(c);
     if (rc) return rc;
     hipStream_t st = pick_stream(c, stream);
     u32 *ctr = next_counter(c);
     const hipError_t e = enc ? fqxv_launch_encode_batch(d_src, d_dst, d_jobs, count, d_order, ctr, d_status,
-                                                        d_out_len, c->ws, grid, st, mode == 2)
+                                                        d_out_len, c->ws, grid, st, mode == 2, c->pre)
                              : fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, ctr, d_status,
                                                         d_out_len, c->ws, grid, st);
     return e == hipSuccess ? FQXV_RANS_OK : FQXV_RANS_ERR_DEVICE;
@@ -433,7 +449,8 @@ int fqxv_run_jobs_host(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_
     const u64 jb = sizeof(fqxv_rans_job) * (u64)count, ob = 4ull * count, sb = 4ull * count, lb = 8ull * count;
     int rc;
     if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, src_len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dst_total)) ||
-        (rc = ensure_buf(&c->job_buf, &c->job_cap, lb + jb + ob + sb)) || (rc = ensure_ws(c, grid)))
+        (rc = ensure_buf(&c->job_buf, &c->job_cap, lb + jb + ob + sb)) || (rc = ensure_ws(c, grid)) ||
+        (mode && (rc = ensure_pre(c))))
         return rc;
     u64 *d_len = (u64 *)c->job_buf;
     fqxv_rans_job *d_jobs = (fqxv_rans_job *)(d_len + count);
@@ -446,7 +463,7 @@ int fqxv_run_jobs_host(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint64_
         return FQXV_RANS_ERR_DEVICE;
     const hipError_t e =
         mode ? fqxv_launch_encode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len,
-                                        c->ws, grid, st, 1)
+                                        c->ws, grid, st, 1, c->pre)
              : fqxv_launch_decode_batch(c->d_in, c->d_out, d_jobs, count, d_order, next_counter(c), d_st, d_len,
                                         c->ws, grid, st);
     if (e != hipSuccess) return FQXV_RANS_ERR_DEVICE;
@@ -480,11 +497,12 @@ int fqxv_jobs_host_packed(fqxv_rans_ctx *c, int mode, const uint8_t *h_src, uint
     const u64 jb = sizeof(fqxv_rans_job) * (u64)count, ob = 4ull * count, sb = 4ull * count, lb = 8ull * count;
     int rc;
     if ((rc = ensure_buf(&c->d_in, &c->d_in_cap, src_len)) || (rc = ensure_buf(&c->d_out, &c->d_out_cap, dst_total)) ||
-        (rc = ensure_buf(&c->job_buf, &c->job_cap, 2 * lb + jb + ob + sb)) || (rc = ensure_ws(c, grid)))
+        (rc = ensure_buf(&c->job_buf, &c->job_cap, 2 * lb + jb + ob + sb)) || (rc = ensure_ws(c, grid)) ||
+        (mode && (rc = ensure_pre(c))))
         return rc;
     auto launch = [&](u32 *ctr, i32 *d_st, u64 *d_len, fqxv_rans_job *d_jobs, u32 *d_order, hipStream_t st) {
         return mode ? fqxv_launch_encode_batch(c->d_in, c->d_out, d_jobs, count, d_order, ctr, d_st, d_len, c->ws,
-                                               grid, st, 1)
+                                               grid, st, 1, c->pre)
                     : fqxv_launch_decode_batch(c->d_in, c->d_out, d_jobs, count, d_order, ctr, d_st, d_len, c->ws,
                                                grid, st);
     };
@@ -604,6 +622,7 @@ int fqxv_rans_graph_create(fqxv_rans_ctx *c, int direction, const uint8_t *d_src
     // The graph bakes in the workspace address: size it for the largest grid
     // any later call on this context can ask for, so it is never reallocated.
     int rc = ensure_ws(c, std::max(c->dec_max, c->enc_max));
+    if (!rc && enc) rc = ensure_pre(c);
     if (rc) return rc;
     auto *g = new (std::nothrow) fqxv_rans_graph();
     if (!g) return FQXV_RANS_ERR_DEVICE;
@@ -613,7 +632,8 @@ int fqxv_rans_graph_create(fqxv_rans_ctx *c, int direction, const uint8_t *d_src
               hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
         const hipError_t e = enc ? fqxv_launch_encode_batch(d_src, d_dst, d_jobs, count, d_order, g->counter,
-                                                            d_status, d_out_len, c->ws, grid, cs, direction == 2)
+                                                            d_status, d_out_len, c->ws, grid, cs, direction == 2,
+                                                            c->pre)
                                  : fqxv_launch_decode_batch(d_src, d_dst, d_jobs, count, d_order, g->counter,
                                                             d_status, d_out_len, c->ws, grid, cs);
         const hipError_t e2 = hipStreamEndCapture(cs, &g->graph);

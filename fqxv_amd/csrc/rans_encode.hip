@@ -64,6 +64,25 @@ static_assert(3ull * 256 * 256 * 4 <= WS_PER_WAVE, "encode workspace");
 // Alphabet sizes whose dense table fits: (a + 1) * a entries.
 __device__ __forceinline__ bool dense_fits(u32 a) { return (a + 1) * a <= TAB_CAP; }
 
+// Histogram pre-pass of a batch's long jobs.  A long job's byte and pair
+// counts on its own wave take several ms while the whole grid is busy (the
+// LDS atomics of every wave compete), at the start of the batch's critical
+// path.  The pre-pass cuts the long jobs (the queue's first PRE_MAX
+// positions of at least PRE_MIN bytes: with the largest-first order, the
+// batch's longest jobs) into PRE_SLICE slices counted by every wave of the
+// grid, then the encode kernel takes the counts instead of counting.
+constexpr u64 PRE_MIN = 2ull << 20;
+constexpr u32 PRE_MAX = 1024;
+constexpr u64 PRE_SLICE = 256ull << 10;
+constexpr u32 PRE_SLOT = 256 + TAB_CAP; // per long job: byte counts, dense pair counts
+struct EncPre {
+    u32 nbig;                       // long jobs: queue positions [0, nbig)
+    u32 nslices;
+    u32 ctr[2];                     // slice queues of the two counting passes
+    u32 slice_start[PRE_MAX + 4];   // first slice of each long job; [nbig..] = nslices
+    u32 slot[PRE_MAX * PRE_SLOT];
+};
+
 // Bytes of p[0..3] that fall in [begin, end), others zero.
 __device__ __forceinline__ u32 load4b(const u8 *p, const u8 *begin, const u8 *end)
 {
@@ -267,41 +286,46 @@ __device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &t
 // memory path sees the few real words, not a store from every lane (a
 // store from every lane, the others into a trash line, was 14% slower on
 // the 10,000-block batch, where the stores of many waves compete).
-__device__ __forceinline__ void store_words(u64 m, u64 addr, u32 x)
+__device__ __forceinline__ void store_words(u64 m, u64 base, u32 voff, u32 x)
 {
+    // Scalar base + per-lane byte offset (the store's saddr form): the
+    // wave-uniform word pointer stays in scalar registers, so the step has
+    // no 64-bit vector address arithmetic.
     u64 saved;
     asm volatile("s_mov_b64 %0, exec\n\t"
                  "s_mov_b64 exec, %1\n\t"
-                 "global_store_short %2, %3, off\n\t"
+                 "global_store_short %2, %3, %4\n\t"
                  "s_mov_b64 exec, %0"
                  : "=&s"(saved)
-                 : "s"(m), "v"(addr), "v"(x)
+                 : "s"(m), "v"(voff), "v"(x), "s"(base)
                  : "memory");
 }
 typedef __attribute__((address_space(1))) u8 gu8;
 typedef __attribute__((address_space(1))) u16 gu16;
-__device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg)
+// x -> (x / f) << 12 | x % f, + cum, given v (x, or x >> 16 after a word)
+// and the magic of f.  v < 2^20 f, so q < 2^20 and q * f < 2^32: the
+// product is a full-rate 24-bit multiply.
+__device__ __forceinline__ u32 step_map(u32 v, u32 f, u32 cum, u32 mg)
+{
+    u32 q = __umulhi(v, mg);
+    u32 rr = v - __umul24(q, f);
+    if (rr >= f) {
+        q++;
+        rr -= f;
+    }
+    return (q << SCALE_BITS) + rr + cum;
+}
+__device__ __forceinline__ void round_step(u32 &x, u64 &wp, u32 fcm, u32 mg)
 {
     const u32 f = fcm >> 12, cum = fcm & 0xfff;
     // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73); inactive
     // lanes (and lanes 32..63) carry f = 4096, which no 32-bit state reaches.
     const bool need = (x >> 20) >= f;
-    // The same test as a 0/1 value (sign of f - 1 - (x >> 20), which lies in
-    // [-4096, 4095]), so the shift and the store address below are
-    // arithmetic, not selects the compiler would lower to exec-mask changes:
-    // the state chain stays on the VALU.
-    const u32 nb = (f - 1u - (x >> 20)) >> 31;
-    const u32 v = x >> (nb << 4);
+    const u32 v = need ? x >> 16 : x;
     const u64 m = ballot(need);
-    wp -= (u32)__popcll(m);
-    store_words(m, (u64)(uintptr_t)(wp + __builtin_amdgcn_mbcnt_lo((u32)m, 0u)), x);
-    u32 q = __umulhi(v, mg);
-    u32 rr = v - q * f;
-    if (rr >= f) {
-        q++;
-        rr -= f;
-    }
-    x = (q << SCALE_BITS) + rr + cum;
+    wp -= 2ull * (u32)__popcll(m);
+    store_words(m, wp, __builtin_amdgcn_mbcnt_lo((u32)m, 0u) << 1, x);
+    x = step_map(v, f, cum, mg);
 }
 
 // The reverse walk over all n positions; returns the number of words, which
@@ -311,8 +335,9 @@ __device__ __forceinline__ void round_step(u32 &x, gu16 *&wp, u32 fcm, u32 mg)
 template <u32 MODE>
 __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u32 &x_out)
 {
-    gu16 *const wtop = (gu16 *)wend;
-    gu16 *wp = wtop;
+    // Wave-uniform (scalar registers): the stores take it as their base.
+    const u64 wtop = uni64((u64)(uintptr_t)wend);
+    u64 wp = wtop; // byte address of the lowest word written so far
     u32 x = RANS_L; // a register for the whole walk, written back at the end
     const u64 nch = (n + CH - 1) / CH;
     const u32 l = lane_id();
@@ -392,7 +417,7 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
         }
     }
     x_out = x;
-    return (u64)(wtop - wp);
+    return (wtop - wp) / 2;
 }
 
 // Tail of every encode: states, then the words slid down behind them.
@@ -427,31 +452,36 @@ __device__ __forceinline__ void round_entry_both(const EncShared &sh, const u8 *
     mg = act ? mv : 1u;
 }
 
-__device__ __forceinline__ void round_step_both(u32 &x, gu16 *&wp1, gu16 *&wp0, u32 fcm, u32 mg)
+// Two stores, each with exec set to one half's emitting lanes and that
+// half's scalar word pointer as the base.
+__device__ __forceinline__ void store_words2(u64 m1, u64 base1, u64 m0, u64 base0, u32 voff, u32 x)
 {
-    const u32 l = lane_id();
+    u64 saved;
+    asm volatile("s_mov_b64 %0, exec\n\t"
+                 "s_mov_b64 exec, %1\n\t"
+                 "global_store_short %3, %4, %2\n\t"
+                 "s_mov_b64 exec, %5\n\t"
+                 "global_store_short %3, %4, %6\n\t"
+                 "s_mov_b64 exec, %0"
+                 : "=&s"(saved)
+                 : "s"(m1), "s"(base1), "v"(voff), "v"(x), "s"(m0), "s"(base0)
+                 : "memory");
+}
+
+__device__ __forceinline__ void round_step_both(u32 &x, u64 &wp1, u64 &wp0, u32 fcm, u32 mg)
+{
     const u32 f = fcm >> 12, cum = fcm & 0xfff;
     const bool need = (x >> 20) >= f;
-    const u32 nb = (f - 1u - (x >> 20)) >> 31;
-    const u32 v = x >> (nb << 4);
+    const u32 v = need ? x >> 16 : x;
     const u64 m = ballot(need);
-    const u32 k1 = (u32)__popc((u32)m), k0 = (u32)__popc((u32)(m >> 32));
-    wp1 -= k1;
-    wp0 -= k0;
-    // Lanes below 32: rank among the order-1 half; from 32: among the order-0
-    // half (the full count minus the order-1 half's words).
-    const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
-    const u32 hi = l >> 5;
-    const u32 rank = below - (k1 & (0u - hi));
-    const u64 w1 = (u64)(uintptr_t)wp1, w0 = (u64)(uintptr_t)wp0;
-    store_words(m, (w1 ^ ((w1 ^ w0) & (0ull - (u64)hi))) + 2ull * rank, x);
-    u32 q = __umulhi(v, mg);
-    u32 rr = v - q * f;
-    if (rr >= f) {
-        q++;
-        rr -= f;
-    }
-    x = (q << SCALE_BITS) + rr + cum;
+    const u64 m1 = m & 0xffffffffull, m0 = m & ~0xffffffffull;
+    wp1 -= 2ull * (u32)__popcll(m1);
+    wp0 -= 2ull * (u32)__popcll(m0);
+    // Rank among the emitting lanes of the lane's own half.
+    const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), 0u) + __builtin_amdgcn_mbcnt_lo((u32)m1, 0u) *
+                     (u32)(lane_id() < NSTATES);
+    store_words2(m1, wp1, m0, wp0, rank << 1, x);
+    x = step_map(v, f, cum, mg);
 }
 
 // Returns the order-1 and order-0 word counts; the words end at wend1 and
@@ -460,8 +490,8 @@ __device__ __forceinline__ void round_step_both(u32 &x, gu16 *&wp1, gu16 *&wp0, 
 __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u32 cols, u32 base0, u8 *wend1,
                                           u8 *wend0, u32 &x_out, u64 &W1, u64 &W0)
 {
-    gu16 *const top1 = (gu16 *)wend1, *const top0 = (gu16 *)wend0;
-    gu16 *wp1 = top1, *wp0 = top0;
+    const u64 top1 = uni64((u64)(uintptr_t)wend1), top0 = uni64((u64)(uintptr_t)wend0);
+    u64 wp1 = top1, wp0 = top0; // byte addresses of each half's lowest word
     u32 x = RANS_L;
     const u64 nch = (n + CH - 1) / CH;
     const u32 l = lane_id();
@@ -522,8 +552,8 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
         }
     }
     x_out = x;
-    W1 = (u64)(top1 - wp1);
-    W0 = (u64)(top0 - wp0);
+    W1 = (top1 - wp1) / 2;
+    W0 = (top0 - wp0) / 2;
 }
 
 // Input dword t (bytes [4t, 4t+4)) as two aligned loads and a byte shift.
@@ -578,8 +608,9 @@ __device__ __forceinline__ void hist0(EncShared &sh, const u8 *in, u64 n)
     __syncthreads();
 }
 
-// Pair counts: row(ctx) x col(sym) for every position (ctx of 0 is 0),
-// with the loads batched as in hist0.
+// Pair counts: row(ctx) x col(sym) for every position, with the loads
+// batched as in hist0.  ctx0 is the context of position 0: 0 for a whole
+// input (scalar.rs:64, 85), the byte before it for a slice of one.
 template <bool DENSE>
 __device__ __forceinline__ void count_pair(EncShared &sh, u32 *gcounts, u32 cols, u32 prev, u32 s)
 {
@@ -590,7 +621,7 @@ __device__ __forceinline__ void count_pair(EncShared &sh, u32 *gcounts, u32 cols
 }
 
 template <bool DENSE>
-__device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols)
+__device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols, u32 ctx0 = 0)
 {
     const u32 l = lane_id();
     const u64 nd = (n + 3) / 4, nbulk = bulk_dwords(n);
@@ -605,7 +636,7 @@ __device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *g
             const u32 a0 = A[t];
             w[u] = __builtin_amdgcn_alignbyte(A[t + 1], a0, ph);
             // Byte 4t - 1: the top byte of input bytes [4t - 4, 4t).
-            pv[u] = t > 0 ? __builtin_amdgcn_alignbyte(a0, A[t - 1], ph) >> 24 : 0u;
+            pv[u] = t > 0 ? __builtin_amdgcn_alignbyte(a0, A[t - 1], ph) >> 24 : ctx0;
         }
 #pragma unroll
         for (u32 u = 0; u < HB; u++) {
@@ -632,7 +663,7 @@ __device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *g
     }
     for (u64 t = t0 + l; t < nd; t += 64) {
         const u32 w = load4(in + 4 * t, in + n);
-        u32 prev = t > 0 ? in[4 * t - 1] : 0u;
+        u32 prev = t > 0 ? in[4 * t - 1] : ctx0;
         const u32 nb = (4 * t + 4 <= n) ? 4 : (u32)(n - 4 * t);
         for (u32 b = 0; b < nb; b++) {
             const u32 s = (w >> (8 * b)) & 0xff;
@@ -689,13 +720,22 @@ struct BestGate {
     u64 l0;
 };
 
+// Counts of a long job taken by the pre-pass (k_enc_pre_count): a slot of
+// PRE_SLOT words, the byte histogram then the dense pair counts.
+__device__ __forceinline__ void load_hist(EncShared &sh, const u32 *pre)
+{
+    for (u32 s = lane_id(); s < 256; s += 64) sh.hist[s] = pre[s];
+    __syncthreads();
+}
+
 // `have_hist`: sh.hist already holds this input's byte histogram (computed
 // by encode_best_job before either order).  `gate`: see BestGate; only the
 // GATED instance (encode-best's order-1 call) has the early choice and the
 // two-order walk compiled in, so the plain encode kernel carries neither.
+// `pre`: this job's pre-pass slot (long jobs), or null.
 template <bool GATED>
 __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb_in, u64 &out_len,
-                          u8 *ws, bool have_hist = false, BestGate *gate = nullptr)
+                          u8 *ws, bool have_hist = false, BestGate *gate = nullptr, const u32 *pre = nullptr)
 {
     const fqxv_rans_job jb = uni_job(jb_in);
     const u32 l = lane_id();
@@ -716,7 +756,12 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     // Words are written downwards from the end of the region, 2-byte
     // aligned (the bound leaves 16 spare bytes), then slid behind the header.
     u8 *wend = (u8 *)((uintptr_t)(o + jb.dst_cap) & ~(uintptr_t)1);
-    if (!have_hist) hist0(sh, in, n);
+    if (!have_hist) {
+        if (pre)
+            load_hist(sh, pre);
+        else
+            hist0(sh, in, n);
+    }
 
     if (order == 0) {
         u32 c[4], f[4];
@@ -757,9 +802,14 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             sh.rowmap[s] = p ? (u8)d : (u8)asz;
             d += p;
         }
-        for (u32 e = l; e < (asz + 1) * asz; e += 64) sh.fc[e] = 0;
-        __syncthreads();
-        hist1<true>(sh, in, n, nullptr, asz);
+        if (pre) {
+            for (u32 e = l; e < (asz + 1) * asz; e += 64) sh.fc[e] = pre[256 + e];
+            __syncthreads();
+        } else {
+            for (u32 e = l; e < (asz + 1) * asz; e += 64) sh.fc[e] = 0;
+            __syncthreads();
+            hist1<true>(sh, in, n, nullptr, asz);
+        }
     } else {
         for (u64 e = l; e < 256 * 256; e += 64) gcounts[e] = 0;
         __threadfence();
@@ -903,7 +953,7 @@ __device__ __forceinline__ u64 encode_bound_dev(u64 n, int order)
 // of the region, order-1 behind it; an order-1 winner slides down to the
 // front.  The stream's first byte records the choice.
 __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans_job &jb_in, u64 &out_len,
-                               u8 *ws)
+                               u8 *ws, const u32 *pre = nullptr)
 {
     const fqxv_rans_job jb = uni_job(jb_in);
     const int tie = jb.order;
@@ -924,7 +974,10 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
         // Byte histogram once; the order-0 size bounds from its normalised
         // table; order-1 tables next, and a walk only for an order that can
         // still be the smaller one.
-        hist0(sh, src + jb.src_off, jb.src_len);
+        if (pre)
+            load_hist(sh, pre);
+        else
+            hist0(sh, src + jb.src_off, jb.src_len);
         const u32 l = lane_id();
         u32 c[4], f[4];
         for (int i = 0; i < 4; i++) c[i] = sh.hist[4 * l + i];
@@ -932,7 +985,7 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
         u64 dn = 0, up = 0;
         for (int i = 0; i < 4; i++) acc_log_bits(c[i], f[i], dn, up);
         BestGate g{size_bounds(dn, up, jb.src_len, O0_HDR), -1, dst + j0.dst_off, j0.dst_cap, 0};
-        err = encode_job<true>(sh, src, dst, j1, l1, ws, true, &g);
+        err = encode_job<true>(sh, src, dst, j1, l1, ws, true, &g, pre);
         if (err) return err;
         decided = g.decided;
         if (decided == 2) {
@@ -961,15 +1014,128 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
 
 } // namespace
 
+// Pre-pass plan: workgroup b clears long-job slot b; wave 0 of workgroup 0
+// finds the long jobs (queue positions 0.. while src_len >= PRE_MIN) and
+// numbers their slices.  Lane l looks at positions 16l .. 16l + 15.
+__global__ __launch_bounds__(256) void k_enc_pre_plan(const fqxv_rans_job *__restrict__ jobs,
+                                                      const u32 *__restrict__ order, u32 count, EncPre *pre)
+{
+    u32 *slot = pre->slot + (u64)blockIdx.x * PRE_SLOT;
+    for (u32 i = threadIdx.x; i < PRE_SLOT; i += 256) slot[i] = 0;
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+    const u32 l = lane_id();
+    const u32 qmax = count < PRE_MAX ? count : PRE_MAX;
+    u32 ns[16];
+    u32 first_short = ~0u;
+#pragma unroll
+    for (u32 i = 0; i < 16; i++) {
+        const u32 q = 16 * l + i;
+        u64 n = 0;
+        if (q < qmax) n = jobs[order ? order[q] : q].src_len;
+        const bool big = q < qmax && n >= PRE_MIN && n < (1ull << 32);
+        if (!big && first_short == ~0u) first_short = q;
+        ns[i] = big ? (u32)((n + PRE_SLICE - 1) / PRE_SLICE) : 0u;
+    }
+    const u64 m = ballot(first_short != ~0u);
+    const u32 nbig = m ? __builtin_amdgcn_readlane(first_short, (u32)__builtin_ctzll(m)) : qmax;
+    u32 tot = 0;
+#pragma unroll
+    for (u32 i = 0; i < 16; i++) {
+        if (16 * l + i >= nbig) ns[i] = 0;
+        tot += ns[i];
+    }
+    u32 at = wave_scan_incl(tot) - tot;
+#pragma unroll
+    for (u32 i = 0; i < 16; i++) {
+        pre->slice_start[16 * l + i] = at;
+        at += ns[i];
+    }
+    const u32 total = uni(__shfl(at, 63));
+    if (l < 4) pre->slice_start[PRE_MAX + l] = total;
+    if (l == 0) {
+        pre->nbig = nbig;
+        pre->nslices = total;
+        pre->ctr[0] = 0;
+        pre->ctr[1] = 0;
+    }
+}
+
+// One counting pass over the long jobs' slices, grid-strided through a
+// queue: PASS 0 the byte counts, PASS 1 the dense pair counts (order-1 jobs,
+// or every job for encode-best, whose alphabet fits the dense table; the
+// pass-0 counts give the dense ids exactly as encode_job forms them).  Each
+// slice is counted in LDS with encode_job's own histogram code and added to
+// its job's slot with global atomics (nonzero counts only).
+template <u32 PASS>
+__global__ __launch_bounds__(64) void k_enc_pre_count(const u8 *__restrict__ src,
+                                                      const fqxv_rans_job *__restrict__ jobs,
+                                                      const u32 *__restrict__ order, EncPre *pre, int best)
+{
+    __shared__ EncShared sh;
+    const u32 l = lane_id();
+    const u32 nbig = uni(pre->nbig), nslices = uni(pre->nslices);
+    for (;;) {
+        u32 sidx = 0;
+        if (l == 0) sidx = atomicAdd(&pre->ctr[PASS], 1u);
+        sidx = uni(__shfl(sidx, 0));
+        if (sidx >= nslices) break;
+        // The long job holding slice sidx: last b with slice_start[b] <= sidx.
+        u32 lo = 0, hi = nbig - 1;
+        while (lo < hi) {
+            const u32 mid = (lo + hi + 1) / 2;
+            if (uni(pre->slice_start[mid]) <= sidx)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const u32 b = lo;
+        const fqxv_rans_job jb = uni_job(jobs[order ? uni(order[b]) : b]);
+        const u8 *in = src + jb.src_off;
+        const u64 k = sidx - uni(pre->slice_start[b]);
+        const u64 p0 = k * PRE_SLICE;
+        const u64 len = jb.src_len - p0 < PRE_SLICE ? jb.src_len - p0 : PRE_SLICE;
+        u32 *slot = pre->slot + (u64)b * PRE_SLOT;
+        if (PASS == 0) {
+            hist0(sh, in + p0, len);
+            for (u32 s = l; s < 256; s += 64)
+                if (sh.hist[s]) atomicAdd(&slot[s], sh.hist[s]);
+        } else {
+            if (!best && jb.order != 1) continue;
+            u32 pres = 0;
+            for (u32 i = 0; i < 4; i++) pres += slot[4 * l + i] != 0;
+            const u32 asz = uni(wave_sum(pres));
+            if (!dense_fits(asz)) continue;
+            const u32 excl = wave_scan_incl(pres) - pres;
+            u32 d = excl;
+            for (u32 i = 0; i < 4; i++) {
+                const u32 s = 4 * l + i;
+                const bool p = slot[s] != 0;
+                sh.colmap[s] = p ? (u8)d : (u8)0;
+                sh.rowmap[s] = p ? (u8)d : (u8)asz;
+                d += p;
+            }
+            const u32 ne = (asz + 1) * asz;
+            for (u32 e = l; e < ne; e += 64) sh.fc[e] = 0;
+            __syncthreads();
+            hist1<true>(sh, in + p0, len, nullptr, asz, p0 ? (u32)in[p0 - 1] : 0u);
+            for (u32 e = l; e < ne; e += 64)
+                if (sh.fc[e]) atomicAdd(&slot[256 + e], sh.fc[e]);
+        }
+        __syncthreads();
+    }
+}
+
 // BEST: encode-best (both orders, the smaller kept) instead of the job's order.
+// `pre` (may be null): the pre-pass counts of the queue's first nbig jobs.
 template <bool BEST>
 __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, u8 *__restrict__ dst,
                                                     const fqxv_rans_job *__restrict__ jobs, u32 count,
                                                     const u32 *__restrict__ order, u32 *counter,
-                                                    i32 *status, u64 *out_len, u8 *ws)
+                                                    i32 *status, u64 *out_len, u8 *ws, const EncPre *pre)
 {
     __shared__ EncShared sh;
     u8 *myws = ws + (u64)blockIdx.x * WS_PER_WAVE;
+    const u32 nbig = pre ? uni(pre->nbig) : 0u;
     for (;;) {
         u32 w = 0;
         if (lane_id() == 0) w = atomicAdd(counter, 1u);
@@ -977,8 +1143,10 @@ __global__ __launch_bounds__(64) void k_rans_encode(const u8 *__restrict__ src, 
         if (w >= count) break;
         const u32 j = order ? uni(order[w]) : w;
         const fqxv_rans_job jb = uni_job(jobs[j]);
+        const u32 *ps = w < nbig ? pre->slot + (u64)w * PRE_SLOT : nullptr;
         u64 ol = 0;
-        const int err = BEST ? encode_best_job(sh, src, dst, jb, ol, myws) : encode_job<false>(sh, src, dst, jb, ol, myws);
+        const int err = BEST ? encode_best_job(sh, src, dst, jb, ol, myws, ps)
+                             : encode_job<false>(sh, src, dst, jb, ol, myws, false, nullptr, ps);
         if (lane_id() == 0) {
             status[j] = err;
             out_len[j] = err ? 0 : ol;
@@ -1004,17 +1172,26 @@ size_t fqxv_enc_shared_bytes() { return sizeof(EncShared); }
 // ------------------------------------------------------------ host launchers
 __global__ void k_reset_counter_enc(u32 *counter) { *counter = 0; }
 
+size_t fqxv_enc_pre_bytes() { return sizeof(EncPre); }
+
+// `pre` (may be null: no pre-pass): a device buffer of fqxv_enc_pre_bytes().
 hipError_t fqxv_launch_encode_batch(const u8 *src, u8 *dst, const fqxv_rans_job *jobs, u32 count,
                                     const u32 *order, u32 *counter, i32 *status, u64 *out_len, u8 *ws,
-                                    u32 grid, hipStream_t st, int best)
+                                    u32 grid, hipStream_t st, int best, void *pre)
 {
     hipLaunchKernelGGL(k_reset_counter_enc, dim3(1), dim3(1), 0, st, counter);
+    EncPre *p = (EncPre *)pre;
+    if (p) {
+        hipLaunchKernelGGL(k_enc_pre_plan, dim3(PRE_MAX), dim3(256), 0, st, jobs, order, count, p);
+        hipLaunchKernelGGL(k_enc_pre_count<0>, dim3(grid), dim3(64), 0, st, src, jobs, order, p, best);
+        hipLaunchKernelGGL(k_enc_pre_count<1>, dim3(grid), dim3(64), 0, st, src, jobs, order, p, best);
+    }
     if (best)
         hipLaunchKernelGGL(k_rans_encode<true>, dim3(grid), dim3(64), 0, st, src, dst, jobs, count, order, counter,
-                           status, out_len, ws);
+                           status, out_len, ws, (const EncPre *)p);
     else
         hipLaunchKernelGGL(k_rans_encode<false>, dim3(grid), dim3(64), 0, st, src, dst, jobs, count, order,
-                           counter, status, out_len, ws);
+                           counter, status, out_len, ws, (const EncPre *)p);
     return hipGetLastError();
 }
 

@@ -219,6 +219,7 @@ def bench_c4(args, coder, dev, oracle):
     rng = np.random.default_rng(0)
     sample = sorted(set(rng.choice(nb, min(24, nb), replace=False).tolist()) | {int(np.argmax(corpus.sizes))})
     check_sample(oracle, corpus, enc, comp_len, sample)
+    enc_tuned = bench_encode_tuned(coder, corpus, arena, enc, comp_len, args)
     best = bench_encode_best(coder, corpus, arena, oracle, sample[:6]) if args.encode_best else None
     dec = coder.decode_batch(enc.dst, enc.dst_off, comp_len, corpus.sizes)
     torch.cuda.synchronize()
@@ -323,7 +324,7 @@ def bench_c4(args, coder, dev, oracle):
         "pcie": pcie,
         "encode_best": best,
         "corpus": corpus, "value": value, "t_step": t_step, "t_kern": t_kern, "n_total": n_total,
-        "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "perblock": pb,
+        "c_total": c_total, "nb": nb, "encode_gbs": n_total / t_enc / 1e9, "encode_tuned": enc_tuned, "perblock": pb,
         "tuned": tuned, "eager_ms": eager_ms, "graph_kernel_ms": t_kern * 1e3,
         "kind_rounds": kind_rounds,
         "largest_o1": {"block": int(jbig), "n": int(corpus.sizes[jbig]), "ms": round(float(lat_big[0]), 3),
@@ -336,6 +337,51 @@ def bench_c4(args, coder, dev, oracle):
     del arena, enc, dec, out
     torch.cuda.empty_cache()
     return res
+
+
+def bench_encode_tuned(coder, corpus, arena, enc, comp_len, args):
+    """The batched encode as the decode is measured: the autotuner picks the
+    persistent grid and queue order by timing graph replays, then the tuned
+    graph is replayed with HIP events around each launch.  Every block's
+    length, and the bytes of a sample of blocks, equal the eager call's
+    (whose sample was checked against the oracle)."""
+    import torch
+    from fqxv_amd import tune
+    sizes, orders = corpus.sizes, corpus.orders
+    caps = coder.encode_bounds(sizes, orders)
+    d_off = np.zeros_like(caps)
+    d_off[1:] = np.cumsum(caps)[:-1]
+    jobs = coder.jobs_array(corpus.offs, sizes, d_off, caps, order=orders)
+    dst = torch.empty(int(caps.sum()) + 64, dtype=torch.uint8, device=arena.device)
+    top = coder.max_waves()
+    grids = sorted({min(top, g) for g in (2048, 1792, 1536, 1280)}, reverse=True)
+    t = tune.autotune(coder, "encode", arena, dst, jobs, sizes, orders, grids=grids,
+                      policies=("lpt-cost", "lpt-size"), use_cache=False, log=log)
+    g = coder.graph("encode", arena, dst, t.jobs(jobs), t.order(coder, sizes, orders, "encode"))
+    stream = torch.cuda.current_stream()
+    g.launch(stream)
+    torch.cuda.synchronize()
+    ev = events(3)
+    dst.fill_(0xEE)
+    for a, b in ev:
+        a.record(stream)
+        g.launch(stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = min(a.elapsed_time(b) for a, b in ev)
+    assert int((g.status != 0).sum()) == 0
+    ol = g.out_len.cpu().numpy().astype(np.uint64)
+    assert np.array_equal(ol, comp_len), "tuned encode lengths differ from the eager call's"
+    rng = np.random.default_rng(7)
+    for j in sorted(set(rng.choice(len(sizes), 16, replace=False).tolist()) | {int(np.argmax(sizes))}):
+        a0, b0, n = int(d_off[j]), int(enc.dst_off[j]), int(ol[j])
+        assert torch.equal(dst[a0:a0 + n], enc.dst[b0:b0 + n]), f"tuned encode block {j} differs"
+    g.close()
+    del dst
+    coder.set_grid(0)
+    gbs = corpus.total / (ms / 1e3) / 1e9
+    log(f"[c4] tuned batched encode {gbs:.1f} GB/s ({ms:.2f} ms, grid {t.grid}, {t.policy})")
+    return {"gbs": gbs, "ms": ms, "grid": t.grid, "policy": t.policy, "table": t.table}
 
 
 def bench_encode_best(coder, corpus, arena, oracle, sample):
@@ -887,7 +933,12 @@ def main():
         "percall": percall,
         "tuner": {"grid": tuned.grid, "policy": tuned.policy, "staging": tuned.staging,
                   "dimensions": ["grid", "policy", "staging"], "table": tuned.table},
-        "encode": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s", "form": "batched, one persistent launch"},
+        "encode": {"value": round(c4["encode_tuned"]["gbs"], 3), "unit": "GB/s",
+                   "form": "graph replay of one batched launch (tuned grid and queue order), HIP events",
+                   "ms": round(c4["encode_tuned"]["ms"], 3), "grid": c4["encode_tuned"]["grid"],
+                   "policy": c4["encode_tuned"]["policy"], "tuner": c4["encode_tuned"]["table"],
+                   "eager_call": {"value": round(c4["encode_gbs"], 3), "unit": "GB/s",
+                                  "form": "fqxv_rans_encode_batch at the default grid, host job setup included"}},
         "encode_best": c4["encode_best"],
         "crc32c": c4["crc32c"],
         "xxh3": c4["xxh3"],

@@ -71,8 +71,12 @@ __device__ __forceinline__ bool dense_fits(u32 a) { return (a + 1) * a <= TAB_CA
 // positions of at least PRE_MIN bytes: with the largest-first order, the
 // batch's longest jobs) into PRE_SLICE slices counted by every wave of the
 // grid, then the encode kernel takes the counts instead of counting.
-constexpr u64 PRE_MIN = 2ull << 20;
-constexpr u32 PRE_MAX = 1024;
+#ifndef PRE_MIN_MIB
+#define PRE_MIN_MIB 2
+#endif
+constexpr u64 PRE_MIN = (u64)PRE_MIN_MIB << 20;
+constexpr u32 PRE_MAX = 2048;
+constexpr u32 PRE_PER_LANE = PRE_MAX / 64; // plan: queue positions per lane
 constexpr u64 PRE_SLICE = 256ull << 10;
 constexpr u32 PRE_SLOT = 256 + TAB_CAP; // per long job: byte counts, dense pair counts
 struct EncPre {
@@ -1016,7 +1020,7 @@ __device__ int encode_best_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv
 
 // Pre-pass plan: workgroup b clears long-job slot b; wave 0 of workgroup 0
 // finds the long jobs (queue positions 0.. while src_len >= PRE_MIN) and
-// numbers their slices.  Lane l looks at positions 16l .. 16l + 15.
+// numbers their slices.  Lane l looks at positions PRE_PER_LANE * l + i.
 __global__ __launch_bounds__(256) void k_enc_pre_plan(const fqxv_rans_job *__restrict__ jobs,
                                                       const u32 *__restrict__ order, u32 count, EncPre *pre)
 {
@@ -1025,11 +1029,11 @@ __global__ __launch_bounds__(256) void k_enc_pre_plan(const fqxv_rans_job *__res
     if (blockIdx.x != 0 || threadIdx.x >= 64) return;
     const u32 l = lane_id();
     const u32 qmax = count < PRE_MAX ? count : PRE_MAX;
-    u32 ns[16];
+    u32 ns[PRE_PER_LANE];
     u32 first_short = ~0u;
 #pragma unroll
-    for (u32 i = 0; i < 16; i++) {
-        const u32 q = 16 * l + i;
+    for (u32 i = 0; i < PRE_PER_LANE; i++) {
+        const u32 q = PRE_PER_LANE * l + i;
         u64 n = 0;
         if (q < qmax) n = jobs[order ? order[q] : q].src_len;
         const bool big = q < qmax && n >= PRE_MIN && n < (1ull << 32);
@@ -1040,14 +1044,14 @@ __global__ __launch_bounds__(256) void k_enc_pre_plan(const fqxv_rans_job *__res
     const u32 nbig = m ? __builtin_amdgcn_readlane(first_short, (u32)__builtin_ctzll(m)) : qmax;
     u32 tot = 0;
 #pragma unroll
-    for (u32 i = 0; i < 16; i++) {
-        if (16 * l + i >= nbig) ns[i] = 0;
+    for (u32 i = 0; i < PRE_PER_LANE; i++) {
+        if (PRE_PER_LANE * l + i >= nbig) ns[i] = 0;
         tot += ns[i];
     }
     u32 at = wave_scan_incl(tot) - tot;
 #pragma unroll
-    for (u32 i = 0; i < 16; i++) {
-        pre->slice_start[16 * l + i] = at;
+    for (u32 i = 0; i < PRE_PER_LANE; i++) {
+        pre->slice_start[PRE_PER_LANE * l + i] = at;
         at += ns[i];
     }
     const u32 total = uni(__shfl(at, 63));

@@ -697,11 +697,24 @@ __device__ __forceinline__ int decode_o1_fast(DecShared &sh, const u8 *s, const 
                     // All last reads in flight before the first is used.
 #pragma unroll
                     for (u32 i = 0; i < GPL; i++) asm volatile("" : "+v"(e3[i]));
+                    if constexpr (REP > 1) {
 #pragma unroll
-                    for (u32 i = 0; i < GPL; i++) dg[i] = e3[i] | (e0[i] << 8) | (e1[i] << 16) | (e2[i] << 24);
+                        for (u32 i = 0; i < GPL; i++) dg[i] = e3[i] | (e0[i] << 8) | (e1[i] << 16) | (e2[i] << 24);
+                    }
                     u32 c = prev, dv = 0, cv = 0;
 #pragma unroll
                     for (u32 g = 0; g < NSTATES / 4; g++) {
+                        if constexpr (REP == 1) {
+                            // One group per chain step: each group's word is
+                            // packed after the step before it (tied to that
+                            // step's result), so the packing fills the
+                            // chain's readlane hazard slots instead of
+                            // delaying its start (K = 41: 495 -> 479 ns per
+                            // round; with replicated context lanes the
+                            // packing up front measured faster).
+                            asm volatile("" : "+v"(e0[g]), "+v"(e1[g]), "+v"(e2[g]), "+v"(e3[g]) : "s"(c));
+                            dg[g] = e3[g] | (e0[g] << 8) | (e1[g] << 16) | (e2[g] << 24);
+                        }
                         cv = writelane(c, 4 * g, cv);
                         // Byte 0 of c is below CW, so adding the lane group
                         // never carries into the lane select's six bits.

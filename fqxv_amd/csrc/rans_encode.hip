@@ -14,7 +14,7 @@
 //      reference's sequential scan does;
 //   3. per (context, symbol) constants: f << 12 | cum and the reciprocal
 //      magic = floor((2^32-1)/f), which gives q0 in {q-1, q} for every
-//      32-bit state, so one correction yields the exact v / f (any exact
+//      32-bit state, so one compare picks the exact step (any exact
 //      division gives the reference's bytes, model.rs:313-365);
 //   4. the reverse walk: lanes 0..31 hold the states; rounds run from the
 //      last to the first; a ballot + v_mbcnt rank places each lane's
@@ -46,9 +46,14 @@ constexpr u32 CH_DW = CH / 4 + 1;        // + 4 leading bytes (context of the fi
 #endif
 constexpr u32 HB = 8;                    // histogram: dwords per lane loaded together
 
+// `tab` holds the counts of a block first (u32 per entry, from the front:
+// sub-histograms, dense pair counts), then its coding table: entry e is the
+// pair {f << 12 | cum, reciprocal magic} at words 2e, 2e + 1, so a walk step
+// reads its entry with one 64-bit load.  The table is written over the
+// counts row by row, from the last row to the first (entry e overwrites
+// counts 2e and 2e + 1, which belong to rows already done).
 struct EncShared {
-    u32 fc[TAB_CAP];        // counts, then f << 12 | cum
-    u32 mg[TAB_CAP];        // reciprocal magic per entry
+    alignas(16) u32 tab[2 * TAB_CAP];
     u32 hist[256];
     u8 colmap[256];         // symbol -> dense column (order-1 dense mode)
     u8 rowmap[256];         // context -> dense row
@@ -56,10 +61,15 @@ struct EncShared {
     u32 stage[2][CH_DW + 12];   // + slack: lanes past the chunk read inside it
 };
 static_assert(sizeof(EncShared) <= 20480, "two one-wave workgroups per SIMD need <= 20 KiB of LDS each");
+__device__ __forceinline__ u32 *counts(EncShared &sh) { return sh.tab; }
+__device__ __forceinline__ uint2 *entries(EncShared &sh) { return (uint2 *)sh.tab; }
+__device__ __forceinline__ const uint2 *entries(const EncShared &sh) { return (const uint2 *)sh.tab; }
 
-// Workspace of one wave (WS_PER_WAVE bytes): order-1 counts, f << 12 | cum
-// and magics for alphabets past the dense table (256 KiB each).
-static_assert(3ull * 256 * 256 * 4 <= WS_PER_WAVE, "encode workspace");
+// Workspace of one wave (WS_PER_WAVE bytes), for alphabets past the dense
+// table: order-1 counts (256 KiB), then the entries {f << 12 | cum, magic}
+// of all 256 x 256 pairs (512 KiB).
+constexpr u64 WS_ENT = 256ull * 256 * 4;
+static_assert(WS_ENT + 256ull * 256 * 8 <= WS_PER_WAVE, "encode workspace");
 
 // Alphabet sizes whose dense table fits: (a + 1) * a entries.
 __device__ __forceinline__ bool dense_fits(u32 a) { return (a + 1) * a <= TAB_CAP; }
@@ -215,14 +225,12 @@ __device__ __forceinline__ void stage_put(u32 *buf, const u32 (&r)[9])
     }
 }
 
-// Encode-walk table access.
+// Encode-walk table access: the walk's MODE is 0 (order-0, entry = symbol),
+// 1 (dense order-1 in LDS, entry = row * cols + column) or 2 (order-1 in the
+// workspace, entry = context * 256 + symbol).
 struct TabRef {
-    const u32 *fc;
-    const u32 *mg;
-    const u8 *rowmap;
-    const u8 *colmap;
-    u32 mode;   // 0: order-0 (index = sym); 1: dense order-1; 2: global order-1
-    u32 cols;   // dense order-1: the alphabet size (row stride)
+    const uint2 *gent;  // mode 2: the workspace entries
+    u32 cols;           // mode 1: the alphabet size (row stride)
 };
 
 
@@ -235,14 +243,6 @@ __device__ __forceinline__ void pin16(u32 (&a)[16])
     asm volatile("" : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]),
                  "+v"(a[7]), "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]),
                  "+v"(a[14]), "+v"(a[15]));
-}
-
-template <u32 MODE>
-__device__ __forceinline__ u32 tab_index_t(const TabRef &t, u32 ctx, u32 sym)
-{
-    if (MODE == 0) return sym;
-    if (MODE == 1) return (u32)t.rowmap[ctx] * t.cols + t.colmap[sym];
-    return ctx * 256 + sym;
 }
 
 // One round's table entry for lane l at position i (staged chunk sb, chunk
@@ -273,63 +273,74 @@ __device__ __forceinline__ void round_entry(const EncShared &sh, const TabRef &t
     else
         e = ctx * 256 + sym;
     e = act ? e : 0u;
-    const u32 fv = MODE == 2 ? tr.fc[e] : sh.fc[e];
-    const u32 mv = MODE == 2 ? tr.mg[e] : sh.mg[e];
-    fcm = act ? fv : (TOTFREQ << 12);
-    mg = act ? mv : 1u;
+    const uint2 en = MODE == 2 ? tr.gent[e] : entries(sh)[e];
+    fcm = act ? en.x : (TOTFREQ << 12);
+    mg = act ? en.y : 1u;
 }
 
-// One encode step of every lane (scalar.rs:120-127 per lane; lanes emit in
-// descending position order within the round).  `wp` (wave-uniform) is the
-// lowest word emitted so far; the round's k words go to [wp - k, wp), the
-// emitting lane of rank r (lanes below it that emit) at wp - k + r, so the
-// highest lane's word comes first in the reference's reversed order.  Only
-// the emitting lanes store: one asm block sets exec to the step's ballot
-// around the store and restores it, so the step has no branch (a branch per
-// step splits a batch into blocks the scheduler cannot interleave) and the
-// memory path sees the few real words, not a store from every lane (a
-// store from every lane, the others into a trash line, was 14% slower on
-// the 10,000-block batch, where the stores of many waves compete).
-__device__ __forceinline__ void store_words(u64 m, u64 base, u32 voff, u32 x)
+// Renormalisation words of one staging chunk.  A chunk of CH positions
+// emits at most CH words (one per step), so its words fall in the 2 CH bytes
+// below the word pointer at the chunk's start.  The stores are raw buffer
+// stores on a descriptor for exactly those bytes (base = word pointer - 2 CH,
+// 2 CH records): a lane that emits nothing in a step stores at offset
+// 2^31, outside the records, and the buffer range check drops it.  So a step
+// stores with no branch and no change of the exec mask (a masked store
+// needs the mask written before the store and restored after it, and every
+// vector instruction behind it in the step waits for that), and the memory
+// path sees only the real words.
+constexpr u32 SINK_BYTES = 2 * CH;
+constexpr u32 SINK_DROP = 0x80000000u;
+struct WordSink {
+    __amdgpu_buffer_rsrc_t rs;
+    u32 off; // byte offset of the lowest word so far from the descriptor's base (wave-uniform)
+};
+__device__ __forceinline__ WordSink sink_below(u64 wp)
 {
-    // Scalar base + per-lane byte offset (the store's saddr form): the
-    // wave-uniform word pointer stays in scalar registers, so the step has
-    // no 64-bit vector address arithmetic.
-    u64 saved;
-    asm volatile("s_mov_b64 %0, exec\n\t"
-                 "s_mov_b64 exec, %1\n\t"
-                 "global_store_short %2, %3, %4\n\t"
-                 "s_mov_b64 exec, %0"
-                 : "=&s"(saved)
-                 : "s"(m), "v"(voff), "v"(x), "s"(base)
-                 : "memory");
+    return WordSink{__builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)(wp - SINK_BYTES), 0, (int)SINK_BYTES,
+                                                      0x00020000),
+                    SINK_BYTES};
 }
+// The word pointer after the chunk's words.
+__device__ __forceinline__ u64 sink_end(u64 wp, const WordSink &sk) { return wp - SINK_BYTES + sk.off; }
+__device__ __forceinline__ void sink_store(const WordSink &sk, bool emit, u32 off, u32 x)
+{
+    __builtin_amdgcn_raw_buffer_store_b16((u16)x, sk.rs, (int)(emit ? off : SINK_DROP), 0, 0);
+}
+
 typedef __attribute__((address_space(1))) u8 gu8;
 typedef __attribute__((address_space(1))) u16 gu16;
-// x -> (x / f) << 12 | x % f, + cum, given v (x, or x >> 16 after a word)
-// and the magic of f.  v < 2^20 f, so q < 2^20 and q * f < 2^32: the
-// product is a full-rate 24-bit multiply.
-__device__ __forceinline__ u32 step_map(u32 v, u32 f, u32 cum, u32 mg)
+// x -> (v / f) << 12 | v % f, + cum, given v (x, or x >> 16 after a word)
+// and the magic of f.  With q = v / f and g = 4096 - f, the result is
+// v + cum + q g.  The magic's quotient q0 is q or q - 1, and q0 = q - 1
+// exactly when v >= (q0 + 1) f, so both candidates v + cum + q0 g and that
+// + g are formed beside the test and one select ends the step: four
+// dependent operations after v (the high multiply, a 24-bit multiply-add,
+// the compare, the select).  v < 2^20 f keeps q < 2^20, so the products are
+// 24-bit multiplies and nothing wraps, except (q0 + 1) f = 2^32 for
+// f = 4096, where g = 0 and both candidates are equal.
+__device__ __forceinline__ u32 step_map(u32 v, u32 f, u32 g, u32 vc, u32 mg)
 {
-    u32 q = __umulhi(v, mg);
-    u32 rr = v - __umul24(q, f);
-    if (rr >= f) {
-        q++;
-        rr -= f;
-    }
-    return (q << SCALE_BITS) + rr + cum;
+    const u32 q0 = __umulhi(v, mg);
+    const u32 a = __umul24(q0, g) + vc;
+    const u32 t = __umul24(q0, f) + f;
+    return v >= t ? a + g : a;
 }
-__device__ __forceinline__ void round_step(u32 &x, u64 &wp, u32 fcm, u32 mg)
+// One encode step of every lane (scalar.rs:120-127 per lane; lanes emit in
+// descending position order within the round).  The round's k words go
+// just below the lowest word so far, the emitting lane of rank r (emitting
+// lanes below it) at offset r among them, so the highest lane's word comes
+// first in the reference's reversed order.
+__device__ __forceinline__ void round_step(u32 &x, WordSink &sk, u32 fcm, u32 mg)
 {
-    const u32 f = fcm >> 12, cum = fcm & 0xfff;
+    const u32 f = fcm >> 12, cum = fcm & 0xfff, g = TOTFREQ - f;
     // x >= x_max = f << 20  <=>  (x >> 20) >= f  (model.rs:73); inactive
     // lanes (and lanes 32..63) carry f = 4096, which no 32-bit state reaches.
     const bool need = (x >> 20) >= f;
     const u32 v = need ? x >> 16 : x;
     const u64 m = ballot(need);
-    wp -= 2ull * (u32)__popcll(m);
-    store_words(m, wp, __builtin_amdgcn_mbcnt_lo((u32)m, 0u) << 1, x);
-    x = step_map(v, f, cum, mg);
+    sk.off -= 2u * (u32)__popcll(m);
+    sink_store(sk, need, sk.off + (__builtin_amdgcn_mbcnt_lo((u32)m, 0u) << 1), x);
+    x = step_map(v, f, g, v + cum, mg);
 }
 
 // The reverse walk over all n positions; returns the number of words, which
@@ -339,17 +350,14 @@ __device__ __forceinline__ void round_step(u32 &x, u64 &wp, u32 fcm, u32 mg)
 template <u32 MODE>
 __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const TabRef &tr, u8 *wend, u32 &x_out)
 {
-    // Wave-uniform (scalar registers): the stores take it as their base.
+    // Wave-uniform (scalar registers): each chunk's word sink is based on it.
     const u64 wtop = uni64((u64)(uintptr_t)wend);
     u64 wp = wtop; // byte address of the lowest word written so far
     u32 x = RANS_L; // a register for the whole walk, written back at the end
     const u64 nch = (n + CH - 1) / CH;
     const u32 l = lane_id();
     const bool lo = l < NSTATES;
-    if (l == 0) {
-        sh.fc[TAB_IDLE] = TOTFREQ << 12;
-        sh.mg[TAB_IDLE] = 1;
-    }
+    if (l == 0) entries(sh)[TAB_IDLE] = make_uint2(TOTFREQ << 12, 1u);
     u32 pf[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
     if (MODE == 1) stage_remap(sh, pf);
@@ -362,12 +370,13 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
         const u64 pend = p0 + CH < n ? p0 + CH : n;
         const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
         u64 r = r_hi + 1; // rounds r-1 down to r_lo remain
+        WordSink sk = sink_below(wp);
         // The block's top round may be partial: it takes the exact step
         // first, so every batch below is full (32 positions in [0, n)).
         if (pend == n && (n % NSTATES) != 0) {
             u32 fcm, mg;
             round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);
-            round_step(x, wp, fcm, mg);
+            round_step(x, sk, fcm, mg);
             r--;
         }
         while (r >= r_lo + WALK_BATCH) {
@@ -382,37 +391,39 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
                 cx[k] = MODE == 0 ? 0u : sb[3 + t];
             }
             pin16(sy);
-            pin16(cx);
+            if (MODE != 0) pin16(cx);
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
                 u32 e = MODE == 0 ? sy[k] : (MODE == 1 ? cx[k] * tr.cols + sy[k] : cx[k] * 256 + sy[k]);
+                uint2 en;
                 if (MODE == 2) {
                     e = lo ? e : 0u;
-                    fv[k] = tr.fc[e];
-                    mv[k] = tr.mg[e];
+                    en = tr.gent[e];
                 } else {
                     e = lo ? e : TAB_IDLE;
-                    fv[k] = sh.fc[e];
-                    mv[k] = sh.mg[e];
+                    en = entries(sh)[e];
                 }
+                fv[k] = en.x;
+                mv[k] = en.y;
             }
             pin16(fv);
             pin16(mv);
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
                 if (MODE == 2)
-                    round_step(x, wp, lo ? fv[k] : (TOTFREQ << 12), lo ? mv[k] : 1u);
+                    round_step(x, sk, lo ? fv[k] : (TOTFREQ << 12), lo ? mv[k] : 1u);
                 else
-                    round_step(x, wp, fv[k], mv[k]);
+                    round_step(x, sk, fv[k], mv[k]);
             }
             r -= WALK_BATCH;
         }
         while (r > r_lo) {
             u32 fcm, mg;
             round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);
-            round_step(x, wp, fcm, mg);
+            round_step(x, sk, fcm, mg);
             r--;
         }
+        wp = sink_end(wp, sk);
         if (ch > 0) {
             if (MODE == 1) stage_remap(sh, pf);
             __syncthreads();
@@ -438,7 +449,7 @@ __device__ void finish(u8 *dst, u64 hdr, u8 *wend, u64 W, u32 x)
 // bounds win outright): lanes 0..31 carry the order-1 states, lanes 32..63
 // the order-0 states of the same positions.  Dense order-1 only: the staged
 // bytes are dense ids, the order-1 entry is ctx * cols + sym and the order-0
-// entry base0 + sym, both in sh.fc / sh.mg.  Each half has its own word
+// entry base0 + sym, both in the LDS table.  Each half has its own word
 // pointer; a lane's rank counts the emitting lanes below it in its half.
 __device__ __forceinline__ void round_entry_both(const EncShared &sh, const u8 *sb, u64 p0, u64 r, u64 n,
                                                  u32 cols, u32 base0, u32 &fcm, u32 &mg)
@@ -451,41 +462,29 @@ __device__ __forceinline__ void round_entry_both(const EncShared &sh, const u8 *
     const u32 ctx = sb[3 + t];
     u32 e = l < NSTATES ? ctx * cols + sym : base0 + sym;
     e = act ? e : 0u;
-    const u32 fv = sh.fc[e], mv = sh.mg[e];
-    fcm = act ? fv : (TOTFREQ << 12);
-    mg = act ? mv : 1u;
+    const uint2 en = entries(sh)[e];
+    fcm = act ? en.x : (TOTFREQ << 12);
+    mg = act ? en.y : 1u;
 }
 
-// Two stores, each with exec set to one half's emitting lanes and that
-// half's scalar word pointer as the base.
-__device__ __forceinline__ void store_words2(u64 m1, u64 base1, u64 m0, u64 base0, u32 voff, u32 x)
+// Each half stores its words through its own sink; the other half's lanes
+// are dropped by the range check.
+__device__ __forceinline__ void round_step_both(u32 &x, WordSink &sk1, WordSink &sk0, u32 fcm, u32 mg)
 {
-    u64 saved;
-    asm volatile("s_mov_b64 %0, exec\n\t"
-                 "s_mov_b64 exec, %1\n\t"
-                 "global_store_short %3, %4, %2\n\t"
-                 "s_mov_b64 exec, %5\n\t"
-                 "global_store_short %3, %4, %6\n\t"
-                 "s_mov_b64 exec, %0"
-                 : "=&s"(saved)
-                 : "s"(m1), "s"(base1), "v"(voff), "v"(x), "s"(m0), "s"(base0)
-                 : "memory");
-}
-
-__device__ __forceinline__ void round_step_both(u32 &x, u64 &wp1, u64 &wp0, u32 fcm, u32 mg)
-{
-    const u32 f = fcm >> 12, cum = fcm & 0xfff;
+    const u32 f = fcm >> 12, cum = fcm & 0xfff, g = TOTFREQ - f;
     const bool need = (x >> 20) >= f;
     const u32 v = need ? x >> 16 : x;
     const u64 m = ballot(need);
     const u64 m1 = m & 0xffffffffull, m0 = m & ~0xffffffffull;
-    wp1 -= 2ull * (u32)__popcll(m1);
-    wp0 -= 2ull * (u32)__popcll(m0);
+    sk1.off -= 2u * (u32)__popcll(m1);
+    sk0.off -= 2u * (u32)__popcll(m0);
     // Rank among the emitting lanes of the lane's own half.
-    const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), 0u) + __builtin_amdgcn_mbcnt_lo((u32)m1, 0u) *
-                     (u32)(lane_id() < NSTATES);
-    store_words2(m1, wp1, m0, wp0, rank << 1, x);
-    x = step_map(v, f, cum, mg);
+    const bool lo = lane_id() < NSTATES;
+    const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), 0u) + __builtin_amdgcn_mbcnt_lo((u32)m1, 0u) * (u32)lo;
+    const u32 o = (lo ? sk1.off : sk0.off) + (rank << 1);
+    sink_store(sk1, need && lo, o, x);
+    sink_store(sk0, need && !lo, o, x);
+    x = step_map(v, f, g, v + cum, mg);
 }
 
 // Returns the order-1 and order-0 word counts; the words end at wend1 and
@@ -511,12 +510,13 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
         const u64 pend = p0 + CH < n ? p0 + CH : n;
         const u64 r_hi = (pend - 1) / NSTATES, r_lo = p0 / NSTATES;
         u64 r = r_hi + 1;
+        WordSink sk1 = sink_below(wp1), sk0 = sink_below(wp0);
         // A partial top round first (exact step), then full batches only:
         // every lane of a full round is active.
         if (pend == n && (n % NSTATES) != 0) {
             u32 fcm, mg;
             round_entry_both(sh, sb, p0, r - 1, n, cols, base0, fcm, mg);
-            round_step_both(x, wp1, wp0, fcm, mg);
+            round_step_both(x, sk1, sk0, fcm, mg);
             r--;
         }
         while (r >= r_lo + WALK_BATCH) {
@@ -533,21 +533,24 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
                 const u32 e = l < NSTATES ? cx[k] * cols + sy[k] : base0 + sy[k];
-                fv[k] = sh.fc[e];
-                mv[k] = sh.mg[e];
+                const uint2 en = entries(sh)[e];
+                fv[k] = en.x;
+                mv[k] = en.y;
             }
             pin16(fv);
             pin16(mv);
 #pragma unroll
-            for (u32 k = 0; k < WALK_BATCH; k++) round_step_both(x, wp1, wp0, fv[k], mv[k]);
+            for (u32 k = 0; k < WALK_BATCH; k++) round_step_both(x, sk1, sk0, fv[k], mv[k]);
             r -= WALK_BATCH;
         }
         while (r > r_lo) {
             u32 fcm, mg;
             round_entry_both(sh, sb, p0, r - 1, n, cols, base0, fcm, mg);
-            round_step_both(x, wp1, wp0, fcm, mg);
+            round_step_both(x, sk1, sk0, fcm, mg);
             r--;
         }
+        wp1 = sink_end(wp1, sk1);
+        wp0 = sink_end(wp0, sk0);
         if (ch > 0) {
             stage_remap(sh, pf);
             __syncthreads();
@@ -568,15 +571,16 @@ __device__ __forceinline__ u64 bulk_dwords(u64 n) { return n >= 8 ? (n - 8) / 4 
 
 // Histogram of in[0..n) into sh.hist.  Lanes count into HSUB sub-histograms
 // (lane % HSUB, rows of 257 so a byte value sits in a different bank in
-// each) held in sh.fc, which holds nothing yet; with few distinct bytes,
+// each) held in the table's counts, which hold nothing yet; with few distinct bytes,
 // fewer lanes then meet on one counter.  The rows are summed at the end.
 constexpr u32 HSUB = 4;
-static_assert(HSUB * 257 <= TAB_CAP, "sub-histograms live in sh.fc");
+static_assert(HSUB * 257 <= 2 * TAB_CAP, "sub-histograms live in the table's counts");
 __device__ __forceinline__ void hist0(EncShared &sh, const u8 *in, u64 n)
 {
     const u32 l = lane_id();
-    u32 *sub = sh.fc + (l % HSUB) * 257;
-    for (u32 s = l; s < HSUB * 257; s += 64) sh.fc[s] = 0;
+    u32 *cnt = counts(sh);
+    u32 *sub = cnt + (l % HSUB) * 257;
+    for (u32 s = l; s < HSUB * 257; s += 64) cnt[s] = 0;
     __syncthreads();
     const u64 nd = (n + 3) / 4, nbulk = bulk_dwords(n);
     const u32 ph = (u32)((uintptr_t)in & 3);
@@ -606,7 +610,7 @@ __device__ __forceinline__ void hist0(EncShared &sh, const u8 *in, u64 n)
     for (u32 s = l; s < 256; s += 64) {
         u32 c = 0;
 #pragma unroll
-        for (u32 k = 0; k < HSUB; k++) c += sh.fc[k * 257 + s];
+        for (u32 k = 0; k < HSUB; k++) c += cnt[k * 257 + s];
         sh.hist[s] = c;
     }
     __syncthreads();
@@ -619,7 +623,7 @@ template <bool DENSE>
 __device__ __forceinline__ void count_pair(EncShared &sh, u32 *gcounts, u32 cols, u32 prev, u32 s)
 {
     if (DENSE)
-        atomicAdd(&sh.fc[(u32)sh.rowmap[prev] * cols + sh.colmap[s]], 1u);
+        atomicAdd(&counts(sh)[(u32)sh.rowmap[prev] * cols + sh.colmap[s]], 1u);
     else
         atomicAdd(&gcounts[prev * 256 + s], 1u);
 }
@@ -653,7 +657,7 @@ __device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *g
 #pragma unroll
                 for (u32 b = 0; b < 4; b++) id[b + 1] = sh.colmap[(w[u] >> (8 * b)) & 0xff];
 #pragma unroll
-                for (u32 b = 0; b < 4; b++) atomicAdd(&sh.fc[id[b] * cols + id[b + 1]], 1u);
+                for (u32 b = 0; b < 4; b++) atomicAdd(&counts(sh)[id[b] * cols + id[b + 1]], 1u);
             } else {
                 u32 prev = pv[u];
 #pragma unroll
@@ -775,12 +779,11 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         const u32 tot = f[0] + f[1] + f[2] + f[3];
         u32 cum = wave_scan_incl(tot) - tot;
         for (int i = 0; i < 4; i++) {
-            sh.fc[4 * l + i] = (f[i] << 12) | cum;
-            sh.mg[4 * l + i] = magic_of(f[i]);
+            entries(sh)[4 * l + i] = make_uint2((f[i] << 12) | cum, magic_of(f[i]));
             cum += f[i];
         }
         __syncthreads();
-        TabRef tr{sh.fc, sh.mg, sh.rowmap, sh.colmap, 0, 0};
+        const TabRef tr{nullptr, 0};
         u32 x;
         const u64 W = walk<0>(sh, in, n, tr, wend, x);
         finish(o, O0_HDR, wend, W, x);
@@ -794,8 +797,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     const u32 asz = uni(wave_sum(pres));
     const bool dense = dense_fits(asz);
     u32 *gcounts = (u32 *)ws;
-    u32 *gfc = (u32 *)(ws + 256ull * 256 * 4);
-    u32 *gmg = (u32 *)(ws + 2 * 256ull * 256 * 4);
+    uint2 *gent = (uint2 *)(ws + WS_ENT);
     if (dense) {
         const u32 excl = wave_scan_incl(pres) - pres;
         u32 d = excl;
@@ -807,10 +809,10 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             d += p;
         }
         if (pre) {
-            for (u32 e = l; e < (asz + 1) * asz; e += 64) sh.fc[e] = pre[256 + e];
+            for (u32 e = l; e < (asz + 1) * asz; e += 64) counts(sh)[e] = pre[256 + e];
             __syncthreads();
         } else {
-            for (u32 e = l; e < (asz + 1) * asz; e += 64) sh.fc[e] = 0;
+            for (u32 e = l; e < (asz + 1) * asz; e += 64) counts(sh)[e] = 0;
             __syncthreads();
             hist1<true>(sh, in, n, nullptr, asz);
         }
@@ -829,7 +831,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             const u32 row = sh.rowmap[ctx];
             const bool valid = sh.hist[ctx] != 0 || ctx == 0;
             if (valid)
-                for (u32 cI = 0; cI < asz; cI++) any |= sh.fc[row * asz + cI];
+                for (u32 cI = 0; cI < asz; cI++) any |= counts(sh)[row * asz + cI];
         } else {
             for (u32 sI = 0; sI < 256; sI++) any |= gcounts[ctx * 256 + sI];
         }
@@ -851,13 +853,18 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
     __syncthreads();
     const u64 hdr = HDR0 + BITMAP + 512ull * K + 4 * NSTATES;
     u64 bits_dn = 0, bits_up = 0;
-    for (u32 d = 0; d < K; d++) {
+    // Dense rows go from the last to the first, each entry written over
+    // counts of rows already done: context 0's own row (the last) first when
+    // symbol 0 is absent, then the present symbols' rows in descending order.
+    const bool row0_last = dense && sh.hist[0] == 0;
+    for (u32 it = 0; it < K; it++) {
+        const u32 d = row0_last ? (it == 0 ? 0u : K - it) : K - 1 - it;
         const u32 ctx = sh.ctx_of[d];
         u32 c[4], f[4];
         for (int i = 0; i < 4; i++) {
             const u32 s = 4 * l + i;
             if (dense)
-                c[i] = sh.hist[s] ? sh.fc[(u32)sh.rowmap[ctx] * asz + sh.colmap[s]] : 0u;
+                c[i] = sh.hist[s] ? counts(sh)[(u32)sh.rowmap[ctx] * asz + sh.colmap[s]] : 0u;
             else
                 c[i] = gcounts[ctx * 256 + s];
         }
@@ -870,15 +877,11 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         __syncthreads(); // all lanes read this row's counts before it is overwritten
         for (int i = 0; i < 4; i++) {
             const u32 s = 4 * l + i;
+            const uint2 en = make_uint2((f[i] << 12) | cum, magic_of(f[i]));
             if (dense) {
-                if (sh.hist[s]) {
-                    const u32 e = (u32)sh.rowmap[ctx] * asz + sh.colmap[s];
-                    sh.fc[e] = (f[i] << 12) | cum;
-                    sh.mg[e] = magic_of(f[i]);
-                }
+                if (sh.hist[s]) entries(sh)[(u32)sh.rowmap[ctx] * asz + sh.colmap[s]] = en;
             } else {
-                gfc[ctx * 256 + s] = (f[i] << 12) | cum;
-                gmg[ctx * 256 + s] = magic_of(f[i]);
+                gent[ctx * 256 + s] = en;
             }
             cum += f[i];
         }
@@ -908,10 +911,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         u32 cum = wave_scan_incl(tot) - tot;
         for (int i = 0; i < 4; i++) {
             const u32 s = 4 * l + i;
-            if (f[i]) {
-                sh.fc[base0 + sh.colmap[s]] = (f[i] << 12) | cum;
-                sh.mg[base0 + sh.colmap[s]] = magic_of(f[i]);
-            }
+            if (f[i]) entries(sh)[base0 + sh.colmap[s]] = make_uint2((f[i] << 12) | cum, magic_of(f[i]));
             cum += f[i];
         }
         __syncthreads();
@@ -932,8 +932,7 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
         return 0;
     }
     if (!dense) __threadfence();
-    TabRef tr = dense ? TabRef{sh.fc, sh.mg, sh.rowmap, sh.colmap, 1u, asz}
-                      : TabRef{gfc, gmg, sh.rowmap, sh.colmap, 2u, 0};
+    const TabRef tr = dense ? TabRef{nullptr, asz} : TabRef{gent, 0};
     u32 x;
     const u64 W = dense ? walk<1>(sh, in, n, tr, wend, x) : walk<2>(sh, in, n, tr, wend, x);
     finish(o, hdr, wend, W, x);
@@ -1119,11 +1118,11 @@ __global__ __launch_bounds__(64) void k_enc_pre_count(const u8 *__restrict__ src
                 d += p;
             }
             const u32 ne = (asz + 1) * asz;
-            for (u32 e = l; e < ne; e += 64) sh.fc[e] = 0;
+            for (u32 e = l; e < ne; e += 64) counts(sh)[e] = 0;
             __syncthreads();
             hist1<true>(sh, in + p0, len, nullptr, asz, p0 ? (u32)in[p0 - 1] : 0u);
             for (u32 e = l; e < ne; e += 64)
-                if (sh.fc[e]) atomicAdd(&slot[256 + e], sh.fc[e]);
+                if (counts(sh)[e]) atomicAdd(&slot[256 + e], counts(sh)[e]);
         }
         __syncthreads();
     }

@@ -62,6 +62,25 @@ struct EncShared {
 };
 static_assert(sizeof(EncShared) <= 20480, "two one-wave workgroups per SIMD need <= 20 KiB of LDS each");
 __device__ __forceinline__ u32 *counts(EncShared &sh) { return sh.tab; }
+// a * b + c on 24-bit a, b as one full-rate multiply-add (the compiler
+// otherwise picks the 64-bit multiply-add for small known operands).
+__device__ __forceinline__ u32 mad24(u32 a, u32 b, u32 c)
+{
+    u32 d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+typedef __attribute__((address_space(3))) const u64 lds_u64;
+// The LDS address of a shared-memory object, and a 64-bit load at one.
+__device__ __forceinline__ u32 lds_addr(const u32 *p)
+{
+    return (u32)(uintptr_t)(__attribute__((address_space(3))) const u32 *)p;
+}
+__device__ __forceinline__ uint2 lds_load2(u32 a)
+{
+    const u64 v = *(lds_u64 *)(uintptr_t)a;
+    return make_uint2((u32)v, (u32)(v >> 32));
+}
 __device__ __forceinline__ uint2 *entries(EncShared &sh) { return (uint2 *)sh.tab; }
 __device__ __forceinline__ const uint2 *entries(const EncShared &sh) { return (const uint2 *)sh.tab; }
 
@@ -343,6 +362,32 @@ __device__ __forceinline__ void round_step(u32 &x, WordSink &sk, u32 fcm, u32 mg
     x = step_map(v, f, g, v + cum, mg);
 }
 
+// The state update of round_step alone; the round's store is left to
+// store_rounds, which takes the state before the step (xo) and v (vo).  A
+// lane emitted a word exactly when v != x: x >= 2^16 always, so x >> 16 < x.
+__device__ __forceinline__ void step_only(u32 &x, u32 fcm, u32 mg, u32 &xo, u32 &vo)
+{
+    const u32 f = fcm >> 12, cum = fcm & 0xfff, g = TOTFREQ - f;
+    const bool need = (x >> 20) >= f;
+    const u32 v = need ? x >> 16 : x;
+    xo = x;
+    vo = v;
+    x = step_map(v, f, g, v + cum, mg);
+}
+
+// The words of a batch of rounds, in round order, as round_step places them.
+// Rounds with xo = vo (a batch not yet run: all zero) store nothing.
+__device__ __forceinline__ void store_rounds(WordSink &sk, const u32 (&xo)[WALK_BATCH], const u32 (&vo)[WALK_BATCH])
+{
+#pragma unroll
+    for (u32 k = 0; k < WALK_BATCH; k++) {
+        const bool need = vo[k] != xo[k];
+        const u64 m = ballot(need);
+        sk.off -= 2u * (u32)__popcll(m);
+        sink_store(sk, need, sk.off + (__builtin_amdgcn_mbcnt_lo((u32)m, 0u) << 1), xo[k]);
+    }
+}
+
 // The reverse walk over all n positions; returns the number of words, which
 // end at `wend` (2-byte aligned).  The table entries of WALK_BATCH rounds are
 // read before their state updates, so the LDS latency is paid once per batch
@@ -358,6 +403,10 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
     const u32 l = lane_id();
     const bool lo = l < NSTATES;
     if (l == 0) entries(sh)[TAB_IDLE] = make_uint2(TOTFREQ << 12, 1u);
+    // LDS address of entry e: e * estride + ebase, which is the idle entry
+    // for every e on lanes 32..63 (one multiply-add, no select).
+    const u32 estride = lo ? (u32)sizeof(uint2) : 0u;
+    const u32 ebase = lds_addr(sh.tab) + (lo ? 0u : TAB_IDLE * (u32)sizeof(uint2));
     u32 pf[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
     if (MODE == 1) stage_remap(sh, pf);
@@ -379,6 +428,11 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
             round_step(x, sk, fcm, mg);
             r--;
         }
+        // The steps of a batch record their stores, which are made while the
+        // next batch's reads are in flight (and after the last batch).
+        u32 xo[WALK_BATCH], vo[WALK_BATCH];
+#pragma unroll
+        for (u32 k = 0; k < WALK_BATCH; k++) xo[k] = vo[k] = 0;
         while (r >= r_lo + WALK_BATCH) {
             // Three phases, each's reads in flight together: the staged
             // bytes, then the table entries they index, then the steps.
@@ -392,16 +446,16 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
             }
             pin16(sy);
             if (MODE != 0) pin16(cx);
+            store_rounds(sk, xo, vo);
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
-                u32 e = MODE == 0 ? sy[k] : (MODE == 1 ? cx[k] * tr.cols + sy[k] : cx[k] * 256 + sy[k]);
+                u32 e = MODE == 0 ? sy[k] : (MODE == 1 ? mad24(cx[k], tr.cols, sy[k]) : cx[k] * 256 + sy[k]);
                 uint2 en;
                 if (MODE == 2) {
                     e = lo ? e : 0u;
                     en = tr.gent[e];
                 } else {
-                    e = lo ? e : TAB_IDLE;
-                    en = entries(sh)[e];
+                    en = lds_load2(__umul24(e, estride) + ebase);
                 }
                 fv[k] = en.x;
                 mv[k] = en.y;
@@ -411,12 +465,13 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
                 if (MODE == 2)
-                    round_step(x, sk, lo ? fv[k] : (TOTFREQ << 12), lo ? mv[k] : 1u);
+                    step_only(x, lo ? fv[k] : (TOTFREQ << 12), lo ? mv[k] : 1u, xo[k], vo[k]);
                 else
-                    round_step(x, sk, fv[k], mv[k]);
+                    step_only(x, fv[k], mv[k], xo[k], vo[k]);
             }
             r -= WALK_BATCH;
         }
+        store_rounds(sk, xo, vo);
         while (r > r_lo) {
             u32 fcm, mg;
             round_entry<MODE>(sh, tr, sb, p0, r - 1, n, fcm, mg);

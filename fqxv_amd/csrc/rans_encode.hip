@@ -44,7 +44,9 @@ constexpr u32 CH_DW = CH / 4 + 1;        // + 4 leading bytes (context of the fi
 #ifndef WALK_BATCH
 #define WALK_BATCH 16                     // rounds whose table entries are read together
 #endif
-constexpr u32 HB = 8;                    // histogram: dwords per lane loaded together
+#ifndef HB
+#define HB 16                             // histogram: dwords per lane loaded together
+#endif
 
 // `tab` holds the counts of a block first (u32 per entry, from the front:
 // sub-histograms, dense pair counts), then its coding table: entry e is the
@@ -626,9 +628,12 @@ __device__ __forceinline__ u64 bulk_dwords(u64 n) { return n >= 8 ? (n - 8) / 4 
 
 // Histogram of in[0..n) into sh.hist.  Lanes count into HSUB sub-histograms
 // (lane % HSUB, rows of 257 so a byte value sits in a different bank in
-// each) held in the table's counts, which hold nothing yet; with few distinct bytes,
-// fewer lanes then meet on one counter.  The rows are summed at the end.
-constexpr u32 HSUB = 4;
+// each; as many as the table's counts hold, which hold nothing yet): with
+// few distinct bytes, fewer lanes then meet on one counter.  The rows are
+// summed at the end.
+#ifndef HSUB
+#define HSUB 14
+#endif
 static_assert(HSUB * 257 <= 2 * TAB_CAP, "sub-histograms live in the table's counts");
 __device__ __forceinline__ void hist0(EncShared &sh, const u8 *in, u64 n)
 {
@@ -674,19 +679,37 @@ __device__ __forceinline__ void hist0(EncShared &sh, const u8 *in, u64 n)
 // Pair counts: row(ctx) x col(sym) for every position, with the loads
 // batched as in hist0.  ctx0 is the context of position 0: 0 for a whole
 // input (scalar.rs:64, 85), the byte before it for a slice of one.
+//
+// Dense: the (cols + 1) x cols table of counts is kept in as many copies as
+// the table's space holds (up to PSUB; lane l counts into copy l % copies),
+// which are summed into the first copy at the end: frequent pairs then put
+// fewer lanes on one counter.  The counts start at zero here.
+// Otherwise: global atomics on the 256 x 256 gcounts (zeroed by the caller).
+#ifndef PSUB
+#define PSUB 16
+#endif
 template <bool DENSE>
-__device__ __forceinline__ void count_pair(EncShared &sh, u32 *gcounts, u32 cols, u32 prev, u32 s)
+__device__ __forceinline__ void count_pair(EncShared &sh, u32 *cnt, u32 cols, u32 prev, u32 s)
 {
     if (DENSE)
-        atomicAdd(&counts(sh)[(u32)sh.rowmap[prev] * cols + sh.colmap[s]], 1u);
+        atomicAdd(&cnt[(u32)sh.rowmap[prev] * cols + sh.colmap[s]], 1u);
     else
-        atomicAdd(&gcounts[prev * 256 + s], 1u);
+        atomicAdd(&cnt[prev * 256 + s], 1u);
 }
 
 template <bool DENSE>
 __device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *gcounts, u32 cols, u32 ctx0 = 0)
 {
     const u32 l = lane_id();
+    const u32 ne = DENSE ? (cols + 1) * cols : 0u;
+    const u32 stride = ne + 1;  // copies start in different banks
+    u32 copies = DENSE ? 2 * TAB_CAP / stride : 1u;
+    copies = copies < PSUB ? copies : PSUB;
+    u32 *cnt = DENSE ? counts(sh) + (l % copies) * stride : gcounts;
+    if (DENSE) {
+        for (u32 e = l; e < copies * stride; e += 64) counts(sh)[e] = 0;
+        __syncthreads();
+    }
     const u64 nd = (n + 3) / 4, nbulk = bulk_dwords(n);
     const u32 ph = (u32)((uintptr_t)in & 3);
     const u32 *A = (const u32 *)(in - ph);
@@ -712,13 +735,13 @@ __device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *g
 #pragma unroll
                 for (u32 b = 0; b < 4; b++) id[b + 1] = sh.colmap[(w[u] >> (8 * b)) & 0xff];
 #pragma unroll
-                for (u32 b = 0; b < 4; b++) atomicAdd(&counts(sh)[id[b] * cols + id[b + 1]], 1u);
+                for (u32 b = 0; b < 4; b++) atomicAdd(&cnt[id[b] * cols + id[b + 1]], 1u);
             } else {
                 u32 prev = pv[u];
 #pragma unroll
                 for (u32 b = 0; b < 4; b++) {
                     const u32 s = (w[u] >> (8 * b)) & 0xff;
-                    count_pair<DENSE>(sh, gcounts, cols, prev, s);
+                    count_pair<DENSE>(sh, cnt, cols, prev, s);
                     prev = s;
                 }
             }
@@ -730,11 +753,21 @@ __device__ __forceinline__ void hist1(EncShared &sh, const u8 *in, u64 n, u32 *g
         const u32 nb = (4 * t + 4 <= n) ? 4 : (u32)(n - 4 * t);
         for (u32 b = 0; b < nb; b++) {
             const u32 s = (w >> (8 * b)) & 0xff;
-            count_pair<DENSE>(sh, gcounts, cols, prev, s);
+            count_pair<DENSE>(sh, cnt, cols, prev, s);
             prev = s;
         }
     }
     __syncthreads();
+    if (DENSE && copies > 1) {
+        // Copy k sits above the first copy's ne counts, so each lane sums
+        // its own entries in place.
+        for (u32 e = l; e < ne; e += 64) {
+            u32 c = counts(sh)[e];
+            for (u32 k = 1; k < copies; k++) c += counts(sh)[k * stride + e];
+            counts(sh)[e] = c;
+        }
+        __syncthreads();
+    }
 }
 
 // Size bounds of a stream before its walk, for encode_best_job's early
@@ -867,8 +900,6 @@ __device__ int encode_job(EncShared &sh, const u8 *src, u8 *dst, const fqxv_rans
             for (u32 e = l; e < (asz + 1) * asz; e += 64) counts(sh)[e] = pre[256 + e];
             __syncthreads();
         } else {
-            for (u32 e = l; e < (asz + 1) * asz; e += 64) counts(sh)[e] = 0;
-            __syncthreads();
             hist1<true>(sh, in, n, nullptr, asz);
         }
     } else {
@@ -1173,8 +1204,6 @@ __global__ __launch_bounds__(64) void k_enc_pre_count(const u8 *__restrict__ src
                 d += p;
             }
             const u32 ne = (asz + 1) * asz;
-            for (u32 e = l; e < ne; e += 64) counts(sh)[e] = 0;
-            __syncthreads();
             hist1<true>(sh, in + p0, len, nullptr, asz, p0 ? (u32)in[p0 - 1] : 0u);
             for (u32 e = l; e < ne; e += 64)
                 if (counts(sh)[e]) atomicAdd(&slot[256 + e], counts(sh)[e]);

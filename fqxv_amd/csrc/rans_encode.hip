@@ -544,6 +544,26 @@ __device__ __forceinline__ void round_step_both(u32 &x, WordSink &sk1, WordSink 
     x = step_map(v, f, g, v + cum, mg);
 }
 
+// store_rounds for the two-order walk: each half through its own sink.
+__device__ __forceinline__ void store_rounds_both(WordSink &sk1, WordSink &sk0, const u32 (&xo)[WALK_BATCH],
+                                                  const u32 (&vo)[WALK_BATCH])
+{
+    const bool lo = lane_id() < NSTATES;
+#pragma unroll
+    for (u32 k = 0; k < WALK_BATCH; k++) {
+        const bool need = vo[k] != xo[k];
+        const u64 m = ballot(need);
+        const u64 m1 = m & 0xffffffffull;
+        sk1.off -= 2u * (u32)__popcll(m1);
+        sk0.off -= 2u * (u32)__popcll(m & ~0xffffffffull);
+        const u32 rank =
+            __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), 0u) + __builtin_amdgcn_mbcnt_lo((u32)m1, 0u) * (u32)lo;
+        const u32 o = (lo ? sk1.off : sk0.off) + (rank << 1);
+        sink_store(sk1, need && lo, o, xo[k]);
+        sink_store(sk0, need && !lo, o, xo[k]);
+    }
+}
+
 // Returns the order-1 and order-0 word counts; the words end at wend1 and
 // wend0 (2-byte aligned); x_out: order-1 states in lanes 0..31, order-0 in
 // lanes 32..63.
@@ -555,6 +575,11 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
     u32 x = RANS_L;
     const u64 nch = (n + CH - 1) / CH;
     const u32 l = lane_id();
+    // Entry of a staged pair: order-1 half ctx * cols + sym, order-0 half
+    // base0 + sym, as (ctx * ecols + sym) * 8 + ebase per lane.
+    const bool lo = l < NSTATES;
+    const u32 ecols = lo ? cols : 0u;
+    const u32 ebase = lds_addr(sh.tab) + (lo ? 0u : base0 * (u32)sizeof(uint2));
     u32 pf[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
     stage_remap(sh, pf);
@@ -576,6 +601,10 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
             round_step_both(x, sk1, sk0, fcm, mg);
             r--;
         }
+        // Stores deferred to the next batch as in walk.
+        u32 xo[WALK_BATCH], vo[WALK_BATCH];
+#pragma unroll
+        for (u32 k = 0; k < WALK_BATCH; k++) xo[k] = vo[k] = 0;
         while (r >= r_lo + WALK_BATCH) {
             u32 sy[WALK_BATCH], cx[WALK_BATCH], fv[WALK_BATCH], mv[WALK_BATCH];
             const u32 tl = (u32)(r * NSTATES - p0) + (l & (NSTATES - 1));
@@ -587,19 +616,20 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
             }
             pin16(sy);
             pin16(cx);
+            store_rounds_both(sk1, sk0, xo, vo);
 #pragma unroll
             for (u32 k = 0; k < WALK_BATCH; k++) {
-                const u32 e = l < NSTATES ? cx[k] * cols + sy[k] : base0 + sy[k];
-                const uint2 en = entries(sh)[e];
+                const uint2 en = lds_load2(mad24(mad24(cx[k], ecols, sy[k]), (u32)sizeof(uint2), ebase));
                 fv[k] = en.x;
                 mv[k] = en.y;
             }
             pin16(fv);
             pin16(mv);
 #pragma unroll
-            for (u32 k = 0; k < WALK_BATCH; k++) round_step_both(x, sk1, sk0, fv[k], mv[k]);
+            for (u32 k = 0; k < WALK_BATCH; k++) step_only(x, fv[k], mv[k], xo[k], vo[k]);
             r -= WALK_BATCH;
         }
+        store_rounds_both(sk1, sk0, xo, vo);
         while (r > r_lo) {
             u32 fcm, mg;
             round_entry_both(sh, sb, p0, r - 1, n, cols, base0, fcm, mg);

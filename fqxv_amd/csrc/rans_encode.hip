@@ -60,7 +60,9 @@ struct EncShared {
     u8 colmap[256];         // symbol -> dense column (order-1 dense mode)
     u8 rowmap[256];         // context -> dense row
     u8 ctx_of[256];         // k-th present context (ascending)
-    u32 stage[2][CH_DW + 12];   // + slack: lanes past the chunk read inside it
+    // + slack: lanes past the chunk read inside it; rows of whole 16-byte
+    // units (stage_put writes 16 bytes per lane).
+    alignas(16) u32 stage[2][(CH_DW + 15) & ~3u];
 };
 static_assert(sizeof(EncShared) <= 20480, "two one-wave workgroups per SIMD need <= 20 KiB of LDS each");
 __device__ __forceinline__ u32 *counts(EncShared &sh) { return sh.tab; }
@@ -212,15 +214,51 @@ __device__ __forceinline__ void put_freqs(u8 *p, const u32 f[4])
     }
 }
 
-// Stage positions [p0 - 4, p0 + CH) of `in` (bytes outside [in, in+n) = 0).
-__device__ __forceinline__ void stage_fetch(const u8 *in, u64 n, u64 p0, u32 (&r)[9])
+// Stage positions [p0 - 4, p0 + CH) of `in` (bytes outside [in, in+n) = 0):
+// CH_DW dwords, lane l holding stage dwords 8l .. 8l + 8 (the ninth only on
+// lane 63, the last dword).  The loads of a chunk are issued a chunk ahead
+// of their use and must not be waited for where they are issued.  A chunk
+// wholly inside the input (every one but the block's first and, rarely, the
+// one below a top chunk of under 4 positions) loads ten aligned dwords per
+// lane with three wide loads and no per-lane branch, kept raw until
+// stage_bytes realigns them; the others take the byte-exact path (bounds per
+// byte, waited at once) into the same registers with no shift.
+struct StageLoad {
+    u32 a[10];
+    u32 ph; // byte shift of the stage's start in its aligned dword
+};
+typedef u32 v4u32 __attribute__((ext_vector_type(4)));
+typedef u32 v2u32 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const v4u32 gu128;
+typedef __attribute__((address_space(1))) const v2u32 gu64;
+__device__ __forceinline__ void stage_fetch(const u8 *in, u64 n, u64 p0, StageLoad &sl)
 {
+    const u32 l = lane_id();
     const u8 *g = in + p0 - 4;
+    if (p0 >= 4 && p0 + CH + 4 <= n) {
+        // Aligned dwords [8l, 8l + 10) from g rounded down: bytes up to
+        // g + 4 * (CH_DW + 1) <= in + p0 + CH + 4, inside [in, in + n).
+        sl.ph = (u32)((uintptr_t)g & 3);
+        const u32 *A = (const u32 *)(g - sl.ph) + 8 * l;
+        const v4u32 v0 = *(gu128 *)(uintptr_t)A, v1 = *(gu128 *)(uintptr_t)(A + 4);
+        const v2u32 v2 = *(gu64 *)(uintptr_t)(A + 8);
+        sl.a[0] = v0.x, sl.a[1] = v0.y, sl.a[2] = v0.z, sl.a[3] = v0.w;
+        sl.a[4] = v1.x, sl.a[5] = v1.y, sl.a[6] = v1.z, sl.a[7] = v1.w;
+        sl.a[8] = v2.x, sl.a[9] = v2.y;
+    } else {
+        sl.ph = 0;
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
-        const u32 t = lane_id() + 64 * i;
-        r[i] = t < CH_DW ? load4b(g + 4 * t, in, in + n) : 0u;
+        for (u32 j = 0; j < 9; j++) {
+            const u32 t = 8 * l + j;
+            sl.a[j] = t < CH_DW ? load4b(g + 4 * t, in, in + n) : 0u;
+        }
+        sl.a[9] = 0;
     }
+}
+__device__ __forceinline__ void stage_bytes(const StageLoad &sl, u32 (&r)[9])
+{
+#pragma unroll
+    for (u32 j = 0; j < 9; j++) r[j] = __builtin_amdgcn_alignbyte(sl.a[j + 1], sl.a[j], sl.ph);
 }
 
 // Dense order-1: every staged byte becomes its dense id (rowmap).  A present
@@ -239,11 +277,11 @@ __device__ __forceinline__ void stage_remap(const EncShared &sh, u32 (&r)[9])
 
 __device__ __forceinline__ void stage_put(u32 *buf, const u32 (&r)[9])
 {
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        const u32 t = lane_id() + 64 * i;
-        if (t < CH_DW) buf[t] = r[i];
-    }
+    const u32 l = lane_id();
+    uint4 *d = (uint4 *)(buf + 8 * l);
+    d[0] = make_uint4(r[0], r[1], r[2], r[3]);
+    d[1] = make_uint4(r[4], r[5], r[6], r[7]);
+    if (8 * l + 8 < CH_DW) buf[8 * l + 8] = r[8];
 }
 
 // Encode-walk table access: the walk's MODE is 0 (order-0, entry = symbol),
@@ -409,10 +447,12 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
     // for every e on lanes 32..63 (one multiply-add, no select).
     const u32 estride = lo ? (u32)sizeof(uint2) : 0u;
     const u32 ebase = lds_addr(sh.tab) + (lo ? 0u : TAB_IDLE * (u32)sizeof(uint2));
-    u32 pf[9];
+    StageLoad pf;
+    u32 sr[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
-    if (MODE == 1) stage_remap(sh, pf);
-    stage_put(sh.stage[(nch - 1) & 1], pf);
+    stage_bytes(pf, sr);
+    if (MODE == 1) stage_remap(sh, sr);
+    stage_put(sh.stage[(nch - 1) & 1], sr);
     __syncthreads();
     for (u64 ch = nch; ch-- > 0;) {
         const u64 p0 = ch * CH;
@@ -482,9 +522,10 @@ __device__ __forceinline__ u64 walk(EncShared &sh, const u8 *in, u64 n, const Ta
         }
         wp = sink_end(wp, sk);
         if (ch > 0) {
-            if (MODE == 1) stage_remap(sh, pf);
+            stage_bytes(pf, sr);
+            if (MODE == 1) stage_remap(sh, sr);
             __syncthreads();
-            stage_put(sh.stage[(ch - 1) & 1], pf);
+            stage_put(sh.stage[(ch - 1) & 1], sr);
             __syncthreads();
         }
     }
@@ -580,10 +621,12 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
     const bool lo = l < NSTATES;
     const u32 ecols = lo ? cols : 0u;
     const u32 ebase = lds_addr(sh.tab) + (lo ? 0u : base0 * (u32)sizeof(uint2));
-    u32 pf[9];
+    StageLoad pf;
+    u32 sr[9];
     stage_fetch(in, n, (nch - 1) * CH, pf);
-    stage_remap(sh, pf);
-    stage_put(sh.stage[(nch - 1) & 1], pf);
+    stage_bytes(pf, sr);
+    stage_remap(sh, sr);
+    stage_put(sh.stage[(nch - 1) & 1], sr);
     __syncthreads();
     for (u64 ch = nch; ch-- > 0;) {
         const u64 p0 = ch * CH;
@@ -639,9 +682,10 @@ __device__ __forceinline__ void walk_both(EncShared &sh, const u8 *in, u64 n, u3
         wp1 = sink_end(wp1, sk1);
         wp0 = sink_end(wp0, sk0);
         if (ch > 0) {
-            stage_remap(sh, pf);
+            stage_bytes(pf, sr);
+            stage_remap(sh, sr);
             __syncthreads();
-            stage_put(sh.stage[(ch - 1) & 1], pf);
+            stage_put(sh.stage[(ch - 1) & 1], sr);
             __syncthreads();
         }
     }

@@ -366,8 +366,6 @@ __device__ __forceinline__ void sink_store(const WordSink &sk, bool emit, u32 of
     __builtin_amdgcn_raw_buffer_store_b16((u16)x, sk.rs, (int)(emit ? off : SINK_DROP), 0, 0);
 }
 
-typedef __attribute__((address_space(1))) u8 gu8;
-typedef __attribute__((address_space(1))) u16 gu16;
 // x -> (v / f) << 12 | v % f, + cum, given v (x, or x >> 16 after a word)
 // and the magic of f.  With q = v / f and g = 4096 - f, the result is
 // v + cum + q g.  The magic's quotient q0 is q or q - 1, and q0 = q - 1

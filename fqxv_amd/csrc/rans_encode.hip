@@ -351,36 +351,37 @@ constexpr u32 SINK_BYTES = 2 * CH;
 constexpr u32 SINK_DROP = 0x80000000u;
 struct WordSink {
     __amdgpu_buffer_rsrc_t rs;
-    u32 off; // byte offset of the lowest word so far from the descriptor's base (wave-uniform)
+    u32 off; // word offset of the lowest word so far from the descriptor's base (wave-uniform)
 };
 __device__ __forceinline__ WordSink sink_below(u64 wp)
 {
     return WordSink{__builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)(wp - SINK_BYTES), 0, (int)SINK_BYTES,
                                                       0x00020000),
-                    SINK_BYTES};
+                    SINK_BYTES / 2};
 }
 // The word pointer after the chunk's words.
-__device__ __forceinline__ u64 sink_end(u64 wp, const WordSink &sk) { return wp - SINK_BYTES + sk.off; }
-__device__ __forceinline__ void sink_store(const WordSink &sk, bool emit, u32 off, u32 x)
+__device__ __forceinline__ u64 sink_end(u64 wp, const WordSink &sk) { return wp - SINK_BYTES + 2ull * sk.off; }
+// Store x's low half as word `w` of the sink (w in words), or drop it.
+__device__ __forceinline__ void sink_store(const WordSink &sk, bool emit, u32 w, u32 x)
 {
-    __builtin_amdgcn_raw_buffer_store_b16((u16)x, sk.rs, (int)(emit ? off : SINK_DROP), 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b16((u16)x, sk.rs, (int)(emit ? w << 1 : SINK_DROP), 0, 0);
 }
 
 // x -> (v / f) << 12 | v % f, + cum, given v (x, or x >> 16 after a word)
 // and the magic of f.  With q = v / f and g = 4096 - f, the result is
 // v + cum + q g.  The magic's quotient q0 is q or q - 1, and q0 = q - 1
-// exactly when v >= (q0 + 1) f, so both candidates v + cum + q0 g and that
-// + g are formed beside the test and one select ends the step: four
-// dependent operations after v (the high multiply, a 24-bit multiply-add,
-// the compare, the select).  v < 2^20 f keeps q < 2^20, so the products are
-// 24-bit multiplies and nothing wraps, except (q0 + 1) f = 2^32 for
-// f = 4096, where g = 0 and both candidates are equal.
-__device__ __forceinline__ u32 step_map(u32 v, u32 f, u32 g, u32 vc, u32 mg)
+// exactly when v >= (q0 + 1) f, so v + q0 g is formed beside that test,
+// which selects g or 0 to add with cum in one three-way add: five dependent
+// operations after v (the high multiply, a 24-bit multiply-add, the
+// compare, the select, the add).  v < 2^20 f keeps q < 2^20, so the
+// products are 24-bit multiplies and nothing wraps, except (q0 + 1) f =
+// 2^32 for f = 4096, where g = 0 and the select adds nothing either way.
+__device__ __forceinline__ u32 step_map(u32 v, u32 f, u32 g, u32 cum, u32 mg)
 {
     const u32 q0 = __umulhi(v, mg);
-    const u32 a = __umul24(q0, g) + vc;
-    const u32 t = __umul24(q0, f) + f;
-    return v >= t ? a + g : a;
+    const u32 a = mad24(q0, g, v);
+    const u32 t = mad24(q0, f, f);
+    return a + (v >= t ? g : 0u) + cum;
 }
 // One encode step of every lane (scalar.rs:120-127 per lane; lanes emit in
 // descending position order within the round).  The round's k words go
@@ -395,9 +396,9 @@ __device__ __forceinline__ void round_step(u32 &x, WordSink &sk, u32 fcm, u32 mg
     const bool need = (x >> 20) >= f;
     const u32 v = need ? x >> 16 : x;
     const u64 m = ballot(need);
-    sk.off -= 2u * (u32)__popcll(m);
-    sink_store(sk, need, sk.off + (__builtin_amdgcn_mbcnt_lo((u32)m, 0u) << 1), x);
-    x = step_map(v, f, g, v + cum, mg);
+    sk.off -= (u32)__popcll(m);
+    sink_store(sk, need, sk.off + __builtin_amdgcn_mbcnt_lo((u32)m, 0u), x);
+    x = step_map(v, f, g, cum, mg);
 }
 
 // The state update of round_step alone; the round's store is left to
@@ -410,7 +411,7 @@ __device__ __forceinline__ void step_only(u32 &x, u32 fcm, u32 mg, u32 &xo, u32 
     const u32 v = need ? x >> 16 : x;
     xo = x;
     vo = v;
-    x = step_map(v, f, g, v + cum, mg);
+    x = step_map(v, f, g, cum, mg);
 }
 
 // The words of a batch of rounds, in round order, as round_step places them.
@@ -421,8 +422,8 @@ __device__ __forceinline__ void store_rounds(WordSink &sk, const u32 (&xo)[WALK_
     for (u32 k = 0; k < WALK_BATCH; k++) {
         const bool need = vo[k] != xo[k];
         const u64 m = ballot(need);
-        sk.off -= 2u * (u32)__popcll(m);
-        sink_store(sk, need, sk.off + (__builtin_amdgcn_mbcnt_lo((u32)m, 0u) << 1), xo[k]);
+        sk.off -= (u32)__popcll(m);
+        sink_store(sk, need, sk.off + __builtin_amdgcn_mbcnt_lo((u32)m, 0u), xo[k]);
     }
 }
 
@@ -572,15 +573,15 @@ __device__ __forceinline__ void round_step_both(u32 &x, WordSink &sk1, WordSink 
     const u32 v = need ? x >> 16 : x;
     const u64 m = ballot(need);
     const u64 m1 = m & 0xffffffffull, m0 = m & ~0xffffffffull;
-    sk1.off -= 2u * (u32)__popcll(m1);
-    sk0.off -= 2u * (u32)__popcll(m0);
+    sk1.off -= (u32)__popcll(m1);
+    sk0.off -= (u32)__popcll(m0);
     // Rank among the emitting lanes of the lane's own half.
     const bool lo = lane_id() < NSTATES;
     const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), 0u) + __builtin_amdgcn_mbcnt_lo((u32)m1, 0u) * (u32)lo;
-    const u32 o = (lo ? sk1.off : sk0.off) + (rank << 1);
+    const u32 o = (lo ? sk1.off : sk0.off) + rank;
     sink_store(sk1, need && lo, o, x);
     sink_store(sk0, need && !lo, o, x);
-    x = step_map(v, f, g, v + cum, mg);
+    x = step_map(v, f, g, cum, mg);
 }
 
 // store_rounds for the two-order walk: each half through its own sink.
@@ -593,11 +594,11 @@ __device__ __forceinline__ void store_rounds_both(WordSink &sk1, WordSink &sk0, 
         const bool need = vo[k] != xo[k];
         const u64 m = ballot(need);
         const u64 m1 = m & 0xffffffffull;
-        sk1.off -= 2u * (u32)__popcll(m1);
-        sk0.off -= 2u * (u32)__popcll(m & ~0xffffffffull);
+        sk1.off -= (u32)__popcll(m1);
+        sk0.off -= (u32)__popcll(m & ~0xffffffffull);
         const u32 rank =
             __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), 0u) + __builtin_amdgcn_mbcnt_lo((u32)m1, 0u) * (u32)lo;
-        const u32 o = (lo ? sk1.off : sk0.off) + (rank << 1);
+        const u32 o = (lo ? sk1.off : sk0.off) + rank;
         sink_store(sk1, need && lo, o, xo[k]);
         sink_store(sk0, need && !lo, o, xo[k]);
     }
